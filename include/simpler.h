@@ -1,0 +1,248 @@
+/*
+ * simpler.h -- C ABI of the MI355X (gfx950) photon-tracing engine.
+ *
+ * This is the drop-in boundary for the reference renderer's hot path:
+ *   photon::Renderer<tvec::TVector3>::renderImage(img, medium, scene, N)
+ *   (reference: renderer/include/photon.h:82-84, renderer/src/photon.cpp:266-340)
+ * and its CUDA backend cuda::CudaRenderer::renderImage
+ *   (reference: renderer/include/cuda_renderer.h:40, renderer/src/cuda_renderer.cu:710-793).
+ *
+ * Plain C: POD structs, pointers and sizes only.  Every entry point returns an
+ * int status (SIMPLER_OK == 0) and never calls exit(); the text of the last
+ * failure on a context is returned by simpler_last_error().  All device
+ * pointers are HIP device pointers (hipMalloc or torch CUDA tensors); host
+ * pointers are marked _host.
+ *
+ * Units: metres for every length (the reference drivers run in metres and its
+ * absolute epsilons -- 1.19209290e-7, the 1e-4 NEE skip, LargeDist = 10000 --
+ * are only meaningful in metres).
+ */
+#ifndef SIMPLER_H_
+#define SIMPLER_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SIMPLER_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------- */
+#define SIMPLER_OK 0
+#define SIMPLER_ERR_INVALID 1   /* bad argument (null, size, range)       */
+#define SIMPLER_ERR_HIP 2       /* a HIP runtime call failed              */
+#define SIMPLER_ERR_STATE 3     /* object in the wrong state              */
+#define SIMPLER_ERR_NOMEM 4     /* device or host allocation failed       */
+
+/* ---- refractive-index field kinds (reference: rif_analytical.h,
+ *      rif_sources.h, rif_interpolated.h; scene.cpp:140-320) ----------- */
+#define SIMPLER_FIELD_BESSEL 0  /* n0 + n_max*s*J_m(k r)cos(m phi)        */
+#define SIMPLER_FIELD_SOURCES 1 /* n0 + sum point-source / travelling wave */
+#define SIMPLER_FIELD_SPLINE 2  /* tricubic B-spline of absolute n         */
+
+/* ---- behaviour flags (defaults reproduce the reference CMake build) --- */
+/* Direct tracing adds its optical length to the photon's running path
+ * length before scattering starts (photon.cpp:40-49,80,331-333). */
+#define SIMPLER_FLAG_DIRECT_LEAK (1u << 0)
+/* Fresnel weights on entry / exit (compiled out in the reference build:
+ * scene.h:157-159, scene.cpp:910-915, photon.cpp:62-67). */
+#define SIMPLER_FLAG_USE_FRESNEL (1u << 1)
+/* Gradient formula of the Bessel field for a general transducer axis; when
+ * clear the reference's axis=+-x component formula (scene.cpp:248-251) is
+ * used. Both agree exactly for axis_uz = +-x. */
+#define SIMPLER_FLAG_GENERAL_AXIS (1u << 2)
+#define SIMPLER_FLAGS_DEFAULT (SIMPLER_FLAG_DIRECT_LEAK | SIMPLER_FLAG_GENERAL_AXIS)
+
+/*
+ * Scene descriptor: everything scn::Scene, med::Medium, pfunc::henyey_greenstein
+ * and photon::Renderer hold for the forward path (scene.h:47-168; medium.h:24-38;
+ * phase.h; photon.h:27-32), in one fixed-layout POD.
+ */
+typedef struct simpler_scene_desc {
+    /* medium block (scn::block, block.cpp:11-23) */
+    float block_l[3];
+    float block_r[3];
+    float ior;                      /* bsdf ior2 (ior1 = 1), scene.h:125    */
+
+    /* textured area source (area_textured_source, scene.cpp:107-137) */
+    float light_origin[3];
+    float light_dir[3];
+    float half_theta_limit;         /* 0 = collimated; > 0 = cone (diffuse) */
+    float light_plane[2];
+    float Li;
+    float emitter_lens_origin[3];
+    float emitter_lens_aperture;
+    float emitter_lens_focal_length;
+    int32_t emitter_lens_active;
+
+    /* orthographic camera (camera.cpp:10-75) */
+    float view_origin[3];
+    float view_dir[3];
+    float view_horizontal[3];
+    float view_plane[2];
+    float pathlength_range[2];      /* (-1,-1) = no time gating            */
+    int32_t use_bounce_decomposition;
+    float sensor_lens_origin[3];
+    float sensor_lens_aperture;
+    float sensor_lens_focal_length;
+    int32_t sensor_lens_active;
+
+    /* refractive-index field (US / rif, scene.h:76-113) */
+    int32_t field_kind;             /* SIMPLER_FIELD_*                      */
+    float n_o;
+    float f_u;                      /* ultrasound frequency (Hz)            */
+    float speed_u;                  /* ultrasound speed (m/s)               */
+    float n_max;                    /* Bessel amplitude                     */
+    float n_clip;                   /* Bessel clip; scaling clamp n_clip/n_max */
+    float phi_min;                  /* duty window start (rad)              */
+    float phi_max;                  /* duty window end (rad)                */
+    int32_t mode;                   /* Bessel order m (0..8)                */
+    float axis_uz[3];               /* transducer axis                      */
+    float axis_ux[3];               /* azimuth reference axis               */
+    float p_u[3];                   /* a point on the axis                  */
+    float er_stepsize;              /* ER step h (m)                        */
+    int32_t precision;              /* decimal refinement levels            */
+    float egap_end_loc_x;           /* n = n_o for x > egap_end_loc_x       */
+    float sgap_begin_loc_x;         /* n = n_o for x < sgap_begin_loc_x     */
+    /* point-source field: per-source data is set with
+     * simpler_scene_set_sources(); time_coupling multiplies the per-photon
+     * duty phase tau in cos(c k r_i + phi_i - time_coupling*tau). */
+    float time_coupling;
+
+    /* medium (med::Medium) and phase function (HG) */
+    float sigma_t;
+    float albedo;
+    float g;
+
+    /* renderer (photon::Renderer) */
+    int32_t max_depth;              /* < 0: unlimited                        */
+    float max_pathlength;           /* < 0: unlimited                        */
+    int32_t use_direct;
+    int32_t use_angular_sampling;
+    uint32_t flags;                 /* SIMPLER_FLAG_*                       */
+} simpler_scene_desc;
+
+/* Event counters of one render (all exact 64-bit sums). */
+typedef struct simpler_stats {
+    uint64_t photons;       /* photons launched                              */
+    uint64_t er_steps;      /* er_step invocations, accepted or refined      */
+    uint64_t scatters;      /* HG scattering events                          */
+    uint64_t nee_splats;    /* NEE contributions that reached the sensor     */
+    uint64_t direct_splats; /* direct contributions that reached the sensor  */
+    uint64_t face_events;   /* dielectric BSDF samples at block faces        */
+    uint64_t segments;      /* march segments (traceTillBlock calls)         */
+    uint64_t reserved;
+} simpler_stats;
+
+typedef struct simpler_ctx simpler_ctx;
+typedef struct simpler_scene simpler_scene;
+
+/* Returns SIMPLER_ABI_VERSION. */
+int simpler_abi_version(void);
+
+/* Fills a descriptor with the defaults of the reference drivers
+ * (renderer/pybindFiles/tests/test_renderer3d_sample.py:9-95,216-245). */
+void simpler_scene_desc_default(simpler_scene_desc *desc);
+
+/* ---- context: one device + one HIP stream; calls on one ctx are not
+ *      thread-safe, separate ctx objects are independent.  Replaces the
+ *      process-global __constant__ state of cuda_renderer.cu:28-48. ------ */
+int simpler_ctx_create(int device, simpler_ctx **out);
+int simpler_ctx_destroy(simpler_ctx *ctx);
+/* Use an external HIP stream (hipStream_t passed as void*); NULL restores the
+ * context's own stream. */
+int simpler_ctx_set_stream(simpler_ctx *ctx, void *hip_stream);
+void *simpler_ctx_get_stream(simpler_ctx *ctx);
+int simpler_ctx_synchronize(simpler_ctx *ctx);
+const char *simpler_last_error(simpler_ctx *ctx);
+
+/* ---- scene (replaces Scene::from / Medium::from deep copies,
+ *      cuda_scene.cuh:808-900): texture is the source's PFM intensity
+ *      (tex_x * tex_y floats, row 0 = top after the PFM flip, image.h:216-223);
+ *      NULL means a 1x1 white texture.  The texel CDF is built on the host
+ *      exactly as area_textured_source.cpp:30-45 / pmf.h:41-99. ---------- */
+int simpler_scene_create(simpler_ctx *ctx, const simpler_scene_desc *desc,
+                         const float *texture_host, int tex_x, int tex_y,
+                         simpler_scene **out);
+int simpler_scene_destroy(simpler_scene *scene);
+/* Cheap mutators (scene.h:179-219: set_f_u, set_n_max, set_n_scaling,
+ * set_phase1, set_phase2): replace every scalar of the descriptor; geometry
+ * sizes of attached tables stay. */
+int simpler_scene_update(simpler_scene *scene, const simpler_scene_desc *desc);
+int simpler_scene_get_desc(const simpler_scene *scene, simpler_scene_desc *out);
+
+/* Point-source field (scene.cpp:140-195; rif_sources.h:156-198):
+ *   n = n_o + sum_{c=1..n_coeff} coeff_weight[c-1] *
+ *               sum_i amp[i] * cos(c*k*r_i + phase[i] - time_coupling*tau) / r_i
+ * with r_i = |p - pos[i]|.  pos_xyz_host holds 3*n_src floats. */
+int simpler_scene_set_sources(simpler_scene *scene, int n_src,
+                              const float *pos_xyz_host, const float *amp_host,
+                              const float *phase_host, int n_coeff,
+                              const float *coeff_weight_host);
+
+/* Tricubic B-spline field (spline.h:88-162,308-448): coeffs_host holds the
+ * prefiltered coefficients, x fastest (index x + N0*(y + N1*z)).  Grid nodes
+ * lie at xmin + i*(xmax-xmin)/(N-1). */
+int simpler_scene_set_spline(simpler_scene *scene, const float *coeffs_host,
+                             const int32_t dims[3], const float xmin[3],
+                             const float xmax[3]);
+
+/* ---- the hot path ---------------------------------------------------- */
+/*
+ * Render photons [photon_offset, photon_offset + n_photons) of a render of
+ * n_total photons into image_dev (float, layout [z][y][x] = z*X*Y + y*X + x,
+ * as Image3::addEnergy image.h:320-324).  The photon weight is
+ * Li*fresnelTrans/n_total (photon.h:76-80), so chunks with offsets sum to the
+ * one-shot render.  accumulate = 0 zeroes image_dev first (renderImage
+ * overwrites its target), 1 adds to it.  Random numbers: Philox-4x32-10 keyed
+ * by seed, counter (photon index, draw index, stream), so the image is a
+ * function of (scene, seed, photon range) only.  Asynchronous on the ctx
+ * stream.  stats_host may be NULL; when given, the call synchronizes and
+ * fills it.
+ */
+int simpler_render(simpler_scene *scene, int64_t n_photons, int64_t n_total,
+                   uint64_t seed, uint64_t photon_offset, float *image_dev,
+                   int x_res, int y_res, int z_res, int accumulate,
+                   simpler_stats *stats_host);
+
+/* Same as simpler_render, but with the image in host memory (copied in when
+ * accumulate = 1, copied back at the end); synchronous, like
+ * cuda::CudaRenderer::renderImage (cuda_renderer.cu:710-742). */
+int simpler_render_host(simpler_scene *scene, int64_t n_photons,
+                        int64_t n_total, uint64_t seed, uint64_t photon_offset,
+                        float *image_host, int x_res, int y_res, int z_res,
+                        int accumulate, simpler_stats *stats_host);
+
+/*
+ * Per-ray record of the direct (unscattered) trace, the transparent-medium
+ * parity mode: for each photon i in [photon_offset, photon_offset+n_rays) the
+ * source is sampled, then the ray is marched to the sensor face exactly as
+ * Renderer::directTracing (photon.cpp:40-87).  Outputs (device, n_rays each):
+ * out_pos (3 floats/ray) = exit point on the block face, out_dir (3) = unit
+ * exit direction inside the medium, out_len = optical path length at the exit
+ * point, out_status = 1 reached the sensor face, 0 lost through another face,
+ * -1 LargeDist exhausted.
+ */
+int simpler_trace_rays(simpler_scene *scene, int64_t n_rays, uint64_t seed,
+                       uint64_t photon_offset, float *out_pos_dev,
+                       float *out_dir_dev, float *out_len_dev,
+                       int32_t *out_status_dev);
+
+/* Read back the counters of the last render on this scene's ctx
+ * (synchronizes). */
+int simpler_read_stats(simpler_ctx *ctx, simpler_stats *stats_host);
+
+/* Tunables of the persistent march kernel (picked by the autotuner):
+ * block_size in {64,128,256,512}, steps_per_chunk >= 1 (march steps between
+ * event phases), blocks_per_cu >= 1.  0 keeps the current value. */
+int simpler_ctx_set_tuning(simpler_ctx *ctx, int block_size,
+                           int steps_per_chunk, int blocks_per_cu);
+int simpler_ctx_get_tuning(simpler_ctx *ctx, int *block_size,
+                           int *steps_per_chunk, int *blocks_per_cu);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIMPLER_H_ */
