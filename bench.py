@@ -1,0 +1,198 @@
+"""Benchmark: photon paths/s of the full-size cylindrical-transducer scene on
+one MI355X, next to the CPU restatement of the reference path.
+
+Workload (a "step"): one chunk of B photons of the 1e9-photon render of the
+10 mm cube (n0 1.3333, Bessel Delta n 2e-5, 848 kHz, 20 % duty window, HG
+sigma_s 1/mm g 0.9, collimated 1 mm disk source, ER step 5 um, 256x256 px x
+128 path-length bins over 0-100 mm, FP32) for each of the two transducer
+modes m = 0 and m = 2.  Chunk k covers photon indices [k B, (k+1) B) of the
+render with weights Li/1e9, so K steps are K/(1e9/B) of the full render; the
+per-photon rate is the same as the one-shot render's.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--photons-per-step B]
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "photon paths/s on 1 MI355X; adjoint vs FD gradient time at 1% rel error"
+N_TOTAL = 10 ** 9
+# algorithmic flops (cost model: + - x = 1, FMA = 2, div/sqrt/rcp = 1,
+# transcendental = 8, J0/J1 = 20): per ER step by Bessel mode, per event
+F_STEP = {0: 92, 1: 126, 2: 148}
+F_EVENT = dict(scatters=80, nee_splats=127, direct_splats=100, face_events=45, photons=40)
+FP32_PEAK_TFLOPS = 157.3
+
+
+def eprint(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_flops(st, mode):
+    return F_STEP[mode] * st["er_steps"] + sum(F_EVENT[k] * st[k] for k in F_EVENT)
+
+
+def gpu_bench(args):
+    import torch
+
+    from simpler_amd import engine, scenes
+
+    ctx = engine.Context(0)
+    if args.tune:
+        ctx.set_tuning(*[int(x) for x in args.tune.split(",")])
+    B = args.photons_per_step
+    modes = [int(m) for m in args.modes.split(",")]
+    work = {}
+    for m in modes:
+        d, tex, res = scenes.bessel_cube_full(m)
+        sc = engine.Scene(ctx, d, tex)
+        X, Y, Z = res
+        img = torch.zeros((Z, Y, X), dtype=torch.float32, device="cuda")
+        work[m] = (sc, img, res)
+    stream = torch.cuda.current_stream()
+
+    def step(k, record):
+        out = []
+        for m in modes:
+            sc, img, res = work[m]
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            _, st = sc.render(B, res, seed=1, offset=k * B, n_total=N_TOTAL, image=img, accumulate=True)
+            e1.record(stream)
+            out.append((m, st, e0, e1))
+        return out
+
+    for w in range(args.warmup):
+        step(w, False)
+        eprint(f"warmup {w + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    recs = []
+    for k in range(args.steps):
+        recs += step(args.warmup + k, True)
+        eprint(f"step {k + 1}/{args.steps} done")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per_mode = {}
+    tot_flops = tot_ms = 0.0
+    tot_steps = 0
+    for m, st, e0, e1 in recs:
+        ms = e0.elapsed_time(e1)
+        f = algorithmic_flops(st, m)
+        pm = per_mode.setdefault(m, dict(photons=0, er_steps=0, kernel_ms=0.0, flops=0.0, launches=0))
+        pm["photons"] += st["photons"]
+        pm["er_steps"] += st["er_steps"]
+        pm["kernel_ms"] += ms
+        pm["flops"] += f
+        pm["launches"] += 1
+        tot_flops += f
+        tot_ms += ms
+        tot_steps += st["er_steps"]
+    photons = sum(pm["photons"] for pm in per_mode.values())
+    res = dict(wall=wall, photons=photons, er_steps=tot_steps, kernel_ms=tot_ms, flops=tot_flops,
+               per_mode={str(m): dict(photon_paths_per_s=pm["photons"] / (pm["kernel_ms"] / 1e3),
+                                      er_steps_per_s=pm["er_steps"] / (pm["kernel_ms"] / 1e3),
+                                      avg_launch_ms=pm["kernel_ms"] / pm["launches"],
+                                      er_steps_per_photon=pm["er_steps"] / pm["photons"],
+                                      tflops=pm["flops"] / (pm["kernel_ms"] / 1e3) / 1e12)
+                         for m, pm in per_mode.items()},
+               tuning=ctx.get_tuning(), launches=len(recs))
+    ctx.close()
+    return res
+
+
+def cpu_baseline(args):
+    from oracle import oracle as orc
+    from simpler_amd import scenes
+
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    d, tex, res = scenes.bessel_cube_full(0)
+    sc = orc.OracleScene(d, tex, fast=True)
+    n = args.cpu_photons
+    t = time.perf_counter()
+    _, st = sc.render(n, res, seed=1, offset=0, n_total=N_TOTAL, precision=32, threads=threads, cutoff=False)
+    dt = time.perf_counter() - t
+    eprint(f"cpu baseline: {n} photons in {dt:.1f}s on {threads} threads")
+    nc = args.cpu_photons_cutoff
+    t = time.perf_counter()
+    _, stc = sc.render(nc, res, seed=1, offset=0, n_total=N_TOTAL, precision=32, threads=threads, cutoff=True)
+    dtc = time.perf_counter() - t
+    eprint(f"cpu with dead-path cutoff: {nc} photons in {dtc:.1f}s")
+    return dict(value=n / dt, unit="photon paths/s", cores=threads, kind="port",
+                sample=(f"photons 0-{n - 1} (seed 1) of the mode-0 1e9-photon render, reference algorithm in "
+                        f"full (4 field evaluations per ER step, glibc jn, no dead-path cutoff), FP32, "
+                        f"g++ -O3 -ffast-math -fopenmp, {threads} threads; {st['er_steps']} ER steps in {dt:.1f} s"),
+                er_steps_per_s=st["er_steps"] / dt,
+                same_algorithm_as_gpu=dict(value=nc / dtc, photons=nc, seconds=dtc,
+                                           er_steps_per_s=stc["er_steps"] / dtc,
+                                           note="same CPU code with the dead-path cutoff the GPU uses"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--photons-per-step", type=int, default=1 << 22)
+    ap.add_argument("--modes", default="0,2")
+    ap.add_argument("--tune", default="", help="block_size,steps_per_chunk,blocks_per_cu")
+    ap.add_argument("--cpu-photons", type=int, default=96)
+    ap.add_argument("--cpu-photons-cutoff", type=int, default=768)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for {max(args.gpus, world)}); photon shards would be "
+              f"rendered independently per GPU and the images reduced once at the end.", file=sys.stderr)
+        sys.exit(2)
+    g = gpu_bench(args)
+    value = g["photons"] / g["wall"]
+    achieved = g["flops"] / (g["kernel_ms"] / 1e3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "photon paths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": g["wall"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("10 mm cube, Bessel transducer modes 0 and 2, dn 2e-5, 848 kHz, 20% duty, sigma_s 1/mm, "
+                         "g 0.9, collimated 1 mm disk, ER step 5 um, 256x256x128 sensor over 0-100 mm, "
+                         "chunks of the 1e9-photon render"),
+            "photons_per_step_per_mode": args.photons_per_step,
+            "modes": [int(m) for m in args.modes.split(",")],
+            "n_total": N_TOTAL,
+            "seed": 1,
+        },
+        "er_steps_per_s": g["er_steps"] / g["wall"],
+        "per_mode": g["per_mode"],
+        "tuning": g["tuning"],
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_render<Bessel> (persistent march + event kernel)",
+                     "avg_launch_ms": g["kernel_ms"] / g["launches"]},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -52,6 +52,12 @@ extern "C" {
  * clear the reference's axis=+-x component formula (scene.cpp:248-251) is
  * used. Both agree exactly for axis_uz = +-x. */
 #define SIMPLER_FLAG_GENERAL_AXIS (1u << 2)
+/* Disable the dead-path cutoff: by default a march stops as soon as its
+ * optical length exceeds pathlength_range.y plus a geometric margin, since
+ * nothing it does afterwards can reach the sensor's time gate
+ * (scene.cpp:793-794).  Each such march draws from its own random
+ * sub-stream, so the image is the same either way; only work is saved. */
+#define SIMPLER_FLAG_NO_DEAD_CUT (1u << 3)
 #define SIMPLER_FLAGS_DEFAULT (SIMPLER_FLAG_DIRECT_LEAK | SIMPLER_FLAG_GENERAL_AXIS)
 
 /*
@@ -132,7 +138,7 @@ typedef struct simpler_stats {
     uint64_t direct_splats; /* direct contributions that reached the sensor  */
     uint64_t face_events;   /* dielectric BSDF samples at block faces        */
     uint64_t segments;      /* march segments (traceTillBlock calls)         */
-    uint64_t reserved;
+    uint64_t aborted;       /* marches abandoned after 2^20 face events      */
 } simpler_stats;
 
 typedef struct simpler_ctx simpler_ctx;
@@ -222,12 +228,21 @@ int simpler_render_host(simpler_scene *scene, int64_t n_photons,
  * out_pos (3 floats/ray) = exit point on the block face, out_dir (3) = unit
  * exit direction inside the medium, out_len = optical path length at the exit
  * point, out_status = 1 reached the sensor face, 0 lost through another face,
- * -1 LargeDist exhausted.
+ * -1 LargeDist exhausted, -2 abandoned (2^20 face events), -4 rejected by
+ * the emitter lens.
  */
 int simpler_trace_rays(simpler_scene *scene, int64_t n_rays, uint64_t seed,
                        uint64_t photon_offset, float *out_pos_dev,
                        float *out_dir_dev, float *out_len_dev,
                        int32_t *out_status_dev);
+
+/* Refractive index n and its gradient at n_pts points (device arrays:
+ * pts 3*n_pts, out_n n_pts, out_grad 3*n_pts) for the per-photon amplitude
+ * scaling s and duty phase tau -- the field evaluation of the march
+ * (scene.h:253-256,337-340). */
+int simpler_field_eval(simpler_scene *scene, int n_pts, const float *pts_dev,
+                       float scaling, float tau, float *out_n_dev,
+                       float *out_grad_dev);
 
 /* Read back the counters of the last render on this scene's ctx
  * (synchronizes). */

@@ -476,8 +476,11 @@ static V3<R> dRIF(const Scene<R> &S, const V3<R> &p, const TimeState<R> &ts) {
 
 /* ------------------------------------------------------------- counters */
 struct Counters {
-    uint64_t photons = 0, steps = 0, scatters = 0, nee = 0, direct = 0, faces = 0, segments = 0;
+    uint64_t photons = 0, steps = 0, scatters = 0, nee = 0, direct = 0, faces = 0, segments = 0, aborted = 0;
 };
+/* A march that takes more than this many face events is abandoned (treated
+ * like a cut march); the reference would loop without bound. */
+static const uint32_t kMaxFaceEvents = 1u << 20;
 
 /* ----------------------------------------------------------- transport */
 template <class R>
@@ -629,8 +632,13 @@ struct Tracer {
         if (dead) return -2;
         distToSensor = disy;
         LargeDist -= disy;
+        uint32_t faces = 0;
         while (true) {
             if (LargeDist < 0) return -1;
+            if (++faces > kMaxFaceEvents) {
+                ++C.aborted;
+                return -2;
+            }
             const V3<R> norm = nearest_face_normal(p);
             if (std::abs(S.camDir.x - norm.x) < K<R>::EPS && std::abs(S.camDir.y - norm.y) < K<R>::EPS &&
                 std::abs(S.camDir.z - norm.z) < K<R>::EPS)
@@ -655,7 +663,12 @@ struct Tracer {
         traceTillBlock(p, d, dist, disy, L);
         if (dead) return false;
         dist -= disy;
+        uint32_t faces = 0;
         while (dist > K<R>::EPS) {
+            if (++faces > kMaxFaceEvents) {
+                ++C.aborted;
+                return false;
+            }
             const V3<R> norm = nearest_face_normal(p);
             const R magnitude = d.length();
             V3<R> d1;
@@ -1034,6 +1047,7 @@ static int render(const Scene<R> &S, int64_t n, uint64_t seed, uint64_t offset, 
             st->direct_splats += c.direct;
             st->face_events += c.faces;
             st->segments += c.segments;
+            st->aborted += c.aborted;
         }
     }
     return 0;

@@ -12,6 +12,7 @@ FIELD_SPLINE = 2
 FLAG_DIRECT_LEAK = 1 << 0
 FLAG_USE_FRESNEL = 1 << 1
 FLAG_GENERAL_AXIS = 1 << 2
+FLAG_NO_DEAD_CUT = 1 << 3
 FLAGS_DEFAULT = FLAG_DIRECT_LEAK | FLAG_GENERAL_AXIS
 
 SIMPLER_OK = 0
@@ -108,8 +109,8 @@ class Stats(C.Structure):
         ("direct_splats", C.c_uint64),
         ("face_events", C.c_uint64),
         ("segments", C.c_uint64),
-        ("reserved", C.c_uint64),
+        ("aborted", C.c_uint64),
     ]
 
     def as_dict(self):
-        return {name: int(getattr(self, name)) for name, _ in self._fields_ if name != "reserved"}
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}

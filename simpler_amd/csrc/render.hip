@@ -1,0 +1,1251 @@
+// render.hip -- the gfx950 photon-tracing engine and its C ABI.
+//
+// One persistent kernel per field kind.  Every lane owns one photon at a time
+// and runs it as a state machine over the reference's call graph
+// (renderImage -> genRay -> directTracing -> scatter -> {movePhoton,
+// addEnergyInParticle -> movePhotonTillSensor} -> traceTillBlock -> er_step;
+// reference renderer/src/photon.cpp:19-136,266-340, scene.cpp:385-946):
+//
+//   loop:
+//     fetch   lanes without a photon take the next photon indices with one
+//             atomic per wave (wave-aggregated work queue), so lanes regenerate
+//             independently and no lane waits for the slowest photon;
+//     events  lanes whose march segment ended run the face / sensor / scatter /
+//             NEE logic until they start their next march segment;
+//     march   up to steps_per_chunk ER steps for every marching lane (the hot
+//             loop: one fused n / grad-n evaluation per step, reused as the
+//             next step's start value).
+//
+// Random numbers are Philox-4x32-10 keyed by the seed with counter (photon,
+// draw/4, sub-stream); sub-stream 0 = main path, 1 = direct trace, 1+depth =
+// NEE at that vertex.  A march whose optical length exceeds the dead-path
+// cutoff (it can no longer reach the sensor's time gate) stops early; because
+// it draws from its own sub-stream the image is unchanged.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/simpler.h"
+#include "fields.h"
+
+namespace simpler {
+
+enum { PH_DIRECT = 0, PH_NEE = 1, PH_FLIGHT = 2 };
+enum { ST_MARCH = 0, ST_SEG_END = 1, ST_NEED = 2, ST_INIT = 3, ST_DONE = 4 };
+enum { C_PHOT = 0, C_STEPS, C_SCAT, C_NEE, C_DIRECT, C_FACE, C_SEG, C_ABORT, C_N };
+constexpr uint32_t kMaxFaceEvents = 1u << 20;  // per photon; see abort counter
+
+struct RenderArgs {
+    float *image;
+    int X, Y, Z;
+    uint64_t offset, n;
+    unsigned long long *work;      // [0] next relative photon index
+    unsigned long long *counters;  // C_N u64
+    float *rec_p, *rec_d, *rec_L;
+    int *rec_s;
+    int spc;
+};
+
+struct Lane {
+    uint32_t pid_lo, pid_hi, stream, draw, main_draw;
+    f3 p, pl, v, g0;
+    float n0, L, Ll;
+    float travelled, seg_dist, h;
+    int prec;
+    uint32_t steps_left;
+    int state, phase;
+    bool dead, first_flight;
+    float acc, large;
+    f3 mp, mv;
+    float mL;
+    int depth;
+    float total_dist, scaling, tau;
+    uint32_t faces;
+    uint32_t c[C_N];
+};
+
+__device__ __forceinline__ float draw_u(Lane &s, const DevScene &S) {
+    uint32_t o[4];
+    philox4x32_10(s.pid_lo, s.pid_hi, s.draw >> 2, s.stream, S.seed_lo, S.seed_hi, o);
+    const uint32_t k = s.draw & 3u;
+    const uint32_t w = k == 0 ? o[0] : (k == 1 ? o[1] : (k == 2 ? o[2] : o[3]));
+    ++s.draw;
+    return to_uniform(w);
+}
+
+template <int FIELD>
+__device__ __forceinline__ void refresh_field(Lane &s, const DevScene &S) {
+    field_eval<FIELD>(S, s.p, s.scaling, s.tau, s.n0, s.g0);
+}
+
+__device__ __forceinline__ void start_segment(Lane &s, const DevScene &S, float dist) {
+    s.travelled = 0.f;
+    s.seg_dist = dist;
+    s.h = S.h0;
+    s.prec = S.precision;
+    const float q = dist / S.h0 + 1.0f;  // traceTillBlock maxsteps (scene.cpp:391)
+    s.steps_left = q >= 4294967040.0f ? 0xFFFFFFFFu : (uint32_t)q;
+    s.dead = false;
+    s.state = ST_MARCH;
+    ++s.c[C_SEG];
+}
+
+__device__ __forceinline__ void end_segment(Lane &s) {
+    s.p = s.p + s.pl;
+    s.pl = mk3(0.f, 0.f, 0.f);
+    s.L += s.Ll;
+    s.Ll = 0.f;
+    s.state = ST_SEG_END;
+}
+
+__device__ __forceinline__ bool inside_block(const DevScene &S, f3 p) {
+    return (p.x - S.blockL.x > -kEps) && (S.blockR.x - p.x > -kEps) && (p.y - S.blockL.y > -kEps) &&
+           (S.blockR.y - p.y > -kEps) && (p.z - S.blockL.z > -kEps) && (S.blockR.z - p.z > -kEps);
+}
+
+// One ER step of traceTillBlock (scene.cpp:395-418) with er_step
+// (scene.cpp:323-330): kick (h/2) grad n(p), drift h v / n(p), kick
+// (h/2) grad n(p').  (n, grad n) at the start point is the previous step's
+// end-point evaluation.
+template <int FIELD>
+__device__ __forceinline__ void march_step(Lane &s, const DevScene &S) {
+    const float h = s.h;
+    const float hh = 0.5f * h;
+    const f3 vh = s.v + hh * s.g0;
+    const float hn = h / s.n0;
+    float px = s.p.x, py = s.p.y, pz = s.p.z;
+    float lx = s.pl.x, ly = s.pl.y, lz = s.pl.z;
+    comp_add(px, lx, hn * vh.x);
+    comp_add(py, ly, hn * vh.y);
+    comp_add(pz, lz, hn * vh.z);
+    const f3 p1 = mk3(px, py, pz);
+    float n1;
+    f3 g1;
+    field_eval<FIELD>(S, p1, s.scaling, s.tau, n1, g1);
+    const f3 v1 = vh + hh * g1;
+    ++s.c[C_STEPS];
+    if (!inside_block(S, p1) || (s.travelled + h) > s.seg_dist) {
+        if (--s.prec < 0) {  // stop past the face / distance (scene.cpp:405-407)
+            s.p = p1;
+            s.pl = mk3(lx, ly, lz);
+            s.v = v1;
+            s.n0 = n1;
+            s.g0 = g1;
+            end_segment(s);
+        } else {  // restore and refine: h/10, 10 more steps (scene.cpp:408-412)
+            s.h = h / 10.0f;
+            s.steps_left = 10;
+        }
+    } else {
+        s.p = p1;
+        s.pl = mk3(lx, ly, lz);
+        s.v = v1;
+        s.n0 = n1;
+        s.g0 = g1;
+        s.travelled += h;
+        comp_add(s.L, s.Ll, h * n1);  // optical length (scene.cpp:416)
+        if (s.L > S.Lcut) {
+            s.dead = true;
+            end_segment(s);
+        } else if (--s.steps_left == 0) {
+            end_segment(s);
+        }
+    }
+}
+
+// scene.cpp:655-676: outward normal of the nearest face (strict <, order
+// L0 R0 L1 R1 L2 R2)
+__device__ __forceinline__ f3 nearest_face(const DevScene &S, f3 p) {
+    float best = 3.40282347e+38f;
+    int axis = 3;
+    float dir = 0.f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float d = fabsf(comp(S.blockL, i) - comp(p, i));
+        if (d < best) {
+            best = d;
+            axis = i;
+            dir = -1.f;
+        }
+        d = fabsf(comp(S.blockR, i) - comp(p, i));
+        if (d < best) {
+            best = d;
+            axis = i;
+            dir = 1.f;
+        }
+    }
+    return mk3(axis == 0 ? dir : 0.f, axis == 1 ? dir : 0.f, axis == 2 ? dir : 0.f);
+}
+
+__device__ __forceinline__ float fresnel_dielectric(float cosI, float cosT, float eta) {
+    if (fabsf(eta - 1.0f) < kEps * fmaxf(1.0f, fabsf(eta))) return 0.f;
+    const float Rs = (cosI - eta * cosT) / (cosI + eta * cosT);
+    const float Rp = (cosT - eta * cosI) / (cosT + eta * cosI);
+    return 0.5f * (Rs * Rs + Rp * Rp);
+}
+
+// bsdf.cpp:25-58 with util.h:59-88 (ior1 = 1, ior2 = scene ior)
+__device__ __forceinline__ f3 bsdf_sample(Lane &s, const DevScene &S, f3 in, f3 nrm) {
+    ++s.c[C_FACE];
+    ++s.faces;
+    if (fabsf(1.0f - S.ior) < kEps) return in;
+    const float eta = dot(in, nrm) < -kEps ? S.ior : 1.0f / S.ior;
+    const float dn = dot(in, nrm);
+    f3 q = dn * nrm;
+    const f3 pp = (in - q) / eta;
+    const f3 outR = in - (2.0f * dn) * nrm;
+    if (len(pp) > 1.0f) return outR;  // total internal reflection, no draw
+    q = normalized(q);
+    const float t = 1.0f - dot(pp, pp);
+    const f3 outT = pp + (t > 0.f ? sqrtf(t) : 0.f) * q;
+    const float F = fresnel_dielectric(fabsf(dot(nrm, in)), fabsf(dot(nrm, outT)), eta);
+    return draw_u(s, S) < F ? outR : outT;
+}
+
+// phase.cpp:39-76 and util.h:17-27
+__device__ __forceinline__ f3 hg_sample(Lane &s, const DevScene &S, f3 in) {
+    const float g = S.g;
+    const float sx = draw_u(s, S), sy = draw_u(s, S);
+    float cosT;
+    if (fabsf(g) < kEps) {
+        cosT = 1.f - 2.f * sx;
+    } else {
+        const float sq = (1.f - g * g) / (1.f - g + 2.f * g * sx);
+        cosT = (1.f + g * g - sq * sq) / (2.f * g);
+    }
+    const float sinT = sqrtf(fmaxf(0.f, 1.f - cosT * cosT));
+    float sp, cp;
+    sincosf(2.0f * kPi * sy, &sp, &cp);
+    f3 c;
+    if (fabsf(in.x) > fabsf(in.y)) {
+        const float il = 1.f / sqrtf(in.x * in.x + in.z * in.z);
+        c = mk3(in.z * il, 0.f, -in.x * il);
+    } else {
+        const float il = 1.f / sqrtf(in.y * in.y + in.z * in.z);
+        c = mk3(0.f, in.z * il, -in.y * il);
+    }
+    const f3 b = cross(c, in);
+    return (sinT * cp) * b + (sinT * sp) * c + cosT * in;
+}
+
+__device__ __forceinline__ float hg_f(const DevScene &S, f3 in, f3 out) {
+    const float g = S.g;
+    const float x = 1.f + g * g - 2.f * g * dot(in, out);
+    return (1.0f / (4.0f * kPi)) * (1.f - g * g) / (x * sqrtf(x));
+}
+
+// scene.cpp:782-829 -> image.h:320-324, with the 4 bilinear taps as
+// device-wide float atomics
+__device__ __forceinline__ void splat(const DevScene &S, const RenderArgs &A, f3 p, float L, int depth,
+                                      float val) {
+    float x = dot(S.cam_h, p) - S.cam_o.y;
+    float y = dot(S.cam_v, p) - S.cam_o.z;
+    if (!(S.nogate || (L > S.pl0 && L < S.pl1))) return;
+    x = (x / S.view_plane[0] + 0.5f) * (float)A.X;
+    y = (y / S.view_plane[1] + 0.5f) * (float)A.Y;
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    if (!(fx0 > -4.f && fx0 < (float)A.X + 4.f && fy0 > -4.f && fy0 < (float)A.Y + 4.f)) return;
+    const int ix = (int)fx0, iy = (int)fy0;
+    int iz;
+    if (S.bounce) {
+        iz = depth;
+    } else if (S.nogate) {
+        iz = 0;
+    } else {
+        const float z = ((L - S.pl0) / (S.pl1 - S.pl0)) * (float)A.Z;
+        const float fz = floorf(z);
+        if (!(fz > -4.f && fz < (float)A.Z + 4.f)) return;
+        iz = (int)fz;
+    }
+    if (iz < 0 || iz >= A.Z) return;
+    const float fx = x - fx0, fy = y - fy0;
+    float *plane = A.image + (size_t)iz * A.X * A.Y;
+    const float w[4] = {val * (1.f - fx) * (1.f - fy), val * fx * (1.f - fy), val * (1.f - fx) * fy, val * fx * fy};
+    const int tx[4] = {ix, ix + 1, ix, ix + 1};
+    const int ty[4] = {iy, iy, iy + 1, iy + 1};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (tx[k] >= 0 && tx[k] < A.X && ty[k] >= 0 && ty[k] < A.Y)
+            unsafeAtomicAdd(plane + (size_t)ty[k] * A.X + tx[k], w[k]);
+}
+
+// lens.cpp:69-97
+__device__ __forceinline__ bool lens_deflect(float ap2, float f, f3 &pos, f3 &dir, float &total) {
+    if (pos.y * pos.y + pos.z * pos.z > ap2) return false;
+    const float invd = -1.f / dir.x;
+    dir.x = -f;
+    dir.y = dir.y * invd * f - pos.y;
+    dir.z = dir.z * invd * f - pos.z;
+    total += f * invd - len(dir);
+    dir = normalized(dir);
+    return true;
+}
+
+// photon.cpp:53-80 / scene.cpp:897-929: leave the medium toward the sensor.
+// Returns false if the sensor lens rejects the ray.
+__device__ __forceinline__ bool exit_to_sensor(const DevScene &S, f3 &p, f3 d, float ior, float &L, float &fore,
+                                               float &fw, f3 &dts) {
+    dts = normalized(d);
+    f3 refr = dts;
+    fw = 1.f;
+    if (ior > 1.f) {
+        refr.x = refr.x / ior;
+        refr = normalized(refr);
+        if (S.use_fresnel) fw = (1.f - fresnel_dielectric(dts.x, refr.x, 1.f / ior)) / ior / ior;
+    }
+    fore = dot(refr, S.cam_dir) / dot(dts, S.cam_dir);
+    float tot = 0.f;
+    if (S.slens_active && !lens_deflect(S.slens_ap2, S.slens_f, p, refr, tot)) return false;
+    const float dist = (S.cam_o.x - p.x) / refr.x;
+    p = p + dist * refr;
+    L += tot + dist;
+    return true;
+}
+
+__device__ __forceinline__ void photon_end(Lane &s) { s.state = ST_NEED; }
+
+template <int FIELD>
+__device__ void flight_start(Lane &s, const DevScene &S, float dist) {
+    s.phase = PH_FLIGHT;
+    s.large = dist;
+    s.faces = 0;
+    start_segment(s, S, dist);
+}
+
+// photon.cpp:19-38
+template <int FIELD>
+__device__ void scatter_once(Lane &s, const DevScene &S) {
+    const float a = S.albedo;
+    if (!((a > 0.f) && ((a >= 1.f) || (draw_u(s, S) < a)))) {
+        photon_end(s);
+        return;
+    }
+    const float mag = len(s.v);
+    const f3 d1 = hg_sample(s, S, s.v / mag);
+    ++s.c[C_SCAT];
+    s.v = mag * d1;
+    const float dist = -S.mfp * logf(draw_u(s, S));
+    flight_start<FIELD>(s, S, dist);
+}
+
+// photon.cpp:97-104: albedo test and first free flight
+template <int FIELD>
+__device__ void scatter_start(Lane &s, const DevScene &S) {
+    const float a = S.albedo;
+    if (!((a > 0.f) && ((a >= 1.f) || (draw_u(s, S) < a)))) {
+        photon_end(s);
+        return;
+    }
+    const float dist = -S.mfp * logf(draw_u(s, S));
+    s.total_dist = dist;
+    s.first_flight = true;
+    flight_start<FIELD>(s, S, dist);
+}
+
+__device__ __forceinline__ void restore_main(Lane &s, float L) {
+    s.p = s.mp;
+    s.v = s.mv;
+    s.L = L;
+    s.stream = 0;
+    s.draw = s.main_draw;
+}
+
+// photon.cpp:111-116 loop head + scene.cpp:866-893 (NEE start)
+template <int FIELD>
+__device__ void vertex(Lane &s, const DevScene &S) {
+    if (!((S.max_depth < 0 || s.depth <= S.max_depth) && (S.max_pathlength < 0.f || s.total_dist <= S.max_pathlength))) {
+        photon_end(s);
+        return;
+    }
+    if (S.use_nee && !((double)(s.p.x - S.cam_o.x) < 1e-4)) {
+        s.mp = s.p;
+        s.mv = s.v;
+        s.mL = s.L;
+        s.main_draw = s.draw;
+        s.stream = 1u + (uint32_t)s.depth;
+        s.draw = 0;
+        // scene.cpp:20-39 + warp.cpp:22-34: uniform hemisphere about -x
+        const float z = draw_u(s, S), s1 = draw_u(s, S);
+        const float tmp = sqrtf(1.f - z * z);
+        float sa, ca;
+        sincosf(2.0f * kPi * s1, &sa, &ca);
+        s.v = s.n0 * mk3(-z, ca * tmp, sa * tmp);
+        s.phase = PH_NEE;
+        s.acc = 0.f;
+        s.large = kLargeDist;
+        s.faces = 0;
+        start_segment(s, S, kLargeDist);
+        return;
+    }
+    scatter_once<FIELD>(s, S);
+}
+
+template <int FIELD, bool RECORD>
+__device__ void direct_done(Lane &s, const DevScene &S, const RenderArgs &A, int res) {
+    if (RECORD) {
+        const size_t i = (size_t)((((uint64_t)s.pid_hi << 32) | s.pid_lo) - A.offset);
+        const f3 d = normalized(s.v);
+        A.rec_p[3 * i] = s.p.x;
+        A.rec_p[3 * i + 1] = s.p.y;
+        A.rec_p[3 * i + 2] = s.p.z;
+        A.rec_d[3 * i] = d.x;
+        A.rec_d[3 * i + 1] = d.y;
+        A.rec_d[3 * i + 2] = d.z;
+        A.rec_L[i] = s.L;
+        A.rec_s[i] = res;
+        photon_end(s);
+        return;
+    }
+    if (res == 1) {
+        f3 p = s.p, dts;
+        float L = s.L, fore, fw;
+        if (exit_to_sensor(S, p, s.v, s.n0, L, fore, fw, dts)) {
+            s.L = L;
+            ++s.c[C_DIRECT];
+            splat(S, A, p, L, 0, S.weight * expf(-S.sigma_t * s.acc) * fw);
+        }
+    }
+    if (res == -2 && S.direct_leak) {  // the leaked length is already dead
+        photon_end(s);
+        return;
+    }
+    restore_main(s, S.direct_leak ? s.L : s.mL);  // photon.cpp:331-333
+    refresh_field<FIELD>(s, S);
+    scatter_start<FIELD>(s, S);
+}
+
+template <int FIELD>
+__device__ void nee_done(Lane &s, const DevScene &S, const RenderArgs &A, int res) {
+    if (res == 1) {
+        f3 p = s.p, dts;
+        float L = s.L, fore, fw;
+        if (exit_to_sensor(S, p, s.v, s.n0, L, fore, fw, dts)) {
+            const f3 din = s.mv / len(s.mv);
+            const float val = S.weight * (2.f * kPi) * expf(-S.sigma_t * s.acc) * hg_f(S, din, dts) * fore * fw;
+            ++s.c[C_NEE];
+            splat(S, A, p, L, s.depth, val);
+        }
+    }
+    restore_main(s, s.mL);
+    refresh_field<FIELD>(s, S);
+    scatter_once<FIELD>(s, S);
+}
+
+// End of a march segment: the face / sensor logic of movePhoton
+// (scene.cpp:719-779) and movePhotonTillSensor (scene.cpp:634-700).
+template <int FIELD, bool RECORD>
+__device__ void seg_end(Lane &s, const DevScene &S, const RenderArgs &A) {
+    if (s.faces > kMaxFaceEvents && !s.dead) {  // no progress: abandon the march
+        ++s.c[C_ABORT];
+        s.dead = true;
+    }
+    if (s.phase == PH_FLIGHT) {
+        if (s.dead) {
+            photon_end(s);
+            return;
+        }
+        s.large -= s.travelled;
+        if (s.large > kEps) {
+            const f3 nrm = nearest_face(S, s.p);
+            const float mag = len(s.v);
+            const f3 d1 = bsdf_sample(s, S, s.v / mag, nrm);
+            if (dot(d1, nrm) < 0.f) {
+                s.v = d1 * mag;
+                start_segment(s, S, s.large);
+            } else {
+                photon_end(s);
+            }
+            return;
+        }
+        if (s.first_flight) {
+            s.depth = 1;
+            s.first_flight = false;
+        } else {
+            ++s.depth;
+        }
+        vertex<FIELD>(s, S);
+        return;
+    }
+    int res;
+    if (s.dead) {
+        res = -2;
+    } else {
+        s.acc += s.travelled;
+        s.large -= s.travelled;
+        if (s.large < 0.f) {
+            res = -1;
+        } else {
+            const f3 nrm = nearest_face(S, s.p);
+            if (fabsf(S.cam_dir.x - nrm.x) < kEps && fabsf(S.cam_dir.y - nrm.y) < kEps &&
+                fabsf(S.cam_dir.z - nrm.z) < kEps) {
+                res = 1;
+            } else {
+                const f3 d1 = bsdf_sample(s, S, s.v, nrm);  // unnormalised v (scene.cpp:686)
+                if (dot(d1, nrm) < 0.f) {
+                    s.v = d1;
+                    start_segment(s, S, s.large);
+                    return;
+                }
+                res = 0;
+            }
+        }
+    }
+    if (s.phase == PH_DIRECT)
+        direct_done<FIELD, RECORD>(s, S, A, res);
+    else
+        nee_done<FIELD>(s, S, A, res);
+}
+
+// pmf.h:109-121
+__device__ __forceinline__ int cdf_sample(const DevScene &S, float u) {
+    const float *c = S.cdf;
+    int lo = 0, hi = S.ncdf;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (c[mid] < u)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    int index = min(S.ncdf - 2, max(0, lo - 1));
+    while (c[index + 1] - c[index] == 0.f && index < S.ncdf - 2) ++index;
+    return index;
+}
+
+// photon.cpp:312-333 up to the direct trace: genRay (scene.cpp:107-137,
+// lens.cpp:19-33), duty-cycle phase, v = n d.
+template <int FIELD, bool RECORD>
+__device__ void photon_init(Lane &s, const DevScene &S, const RenderArgs &A) {
+    ++s.c[C_PHOT];
+    s.stream = 0;
+    s.draw = 0;
+    s.faces = 0;
+    s.pl = mk3(0.f, 0.f, 0.f);
+    s.Ll = 0.f;
+    f3 pos = S.light_origin;
+    const int pixel = cdf_sample(S, draw_u(s, S));
+    const int py = pixel / S.tex_x;
+    const int px = pixel - py * S.tex_x;
+    pos.y += -S.plane[0] / 2.0f + (float)px * S.pixsize[0] + draw_u(s, S) * S.pixsize[0];
+    pos.z += -S.plane[1] / 2.0f + (float)py * S.pixsize[1] + draw_u(s, S) * S.pixsize[1];
+    const float z = draw_u(s, S) * (1.f - S.ct) + S.ct;
+    const float zt = sqrtf(1.f - z * z);
+    const float phi = draw_u(s, S) * 2.f * kPi;
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    f3 dir = mk3(-z, zt * cp, zt * sp);
+    float total = 0.f;
+    const float dl = -(pos.x - S.elens_x) / dir.x;
+    pos = pos + dl * dir;
+    total += dl;
+    if (S.elens_active && !lens_deflect(S.elens_ap2, S.elens_f, pos, dir, total)) {
+        if (RECORD) {
+            s.p = pos;
+            s.v = dir;
+            s.L = total;
+            s.mL = total;
+            direct_done<FIELD, RECORD>(s, S, A, -4);
+        } else {
+            photon_end(s);
+        }
+        return;
+    }
+    s.tau = S.phi_min + S.phi_range * draw_u(s, S);
+    s.scaling = fmaxf(fminf(sinf(s.tau), S.n_max_scaling), -S.n_max_scaling);
+    s.p = pos;
+    refresh_field<FIELD>(s, S);
+    s.v = s.n0 * dir;
+    s.L = total;
+    if (S.use_direct || RECORD) {
+        s.mp = s.p;
+        s.mv = s.v;
+        s.mL = s.L;
+        s.main_draw = s.draw;
+        s.stream = 1;
+        s.draw = 0;
+        s.phase = PH_DIRECT;
+        s.acc = 0.f;
+        s.large = kLargeDist;
+        s.faces = 0;
+        start_segment(s, S, kLargeDist);
+    } else {
+        scatter_start<FIELD>(s, S);
+    }
+}
+
+template <int FIELD, bool RECORD>
+__global__ void __launch_bounds__(256) k_render(DevScene S, RenderArgs A) {
+    Lane s;
+#pragma unroll
+    for (int i = 0; i < C_N; ++i) s.c[i] = 0;
+    s.state = ST_NEED;
+    s.dead = false;
+    const int lane = __lane_id();
+    while (true) {
+        // fetch: one atomic per wave for all lanes that need a photon
+        const unsigned long long need = __ballot(s.state == ST_NEED);
+        if (need) {
+            const int leader = __ffsll((long long)need) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(A.work, (unsigned long long)__popcll(need));
+            base = __shfl(base, leader);
+            if (s.state == ST_NEED) {
+                const unsigned long long rank = __popcll(need & ((1ull << lane) - 1ull));
+                const unsigned long long idx = base + rank;
+                if (idx < A.n) {
+                    const uint64_t pid = A.offset + idx;
+                    s.pid_lo = (uint32_t)pid;
+                    s.pid_hi = (uint32_t)(pid >> 32);
+                    s.state = ST_INIT;
+                } else {
+                    s.state = ST_DONE;
+                }
+            }
+        }
+        if (s.state == ST_INIT) photon_init<FIELD, RECORD>(s, S, A);
+        if (s.state == ST_SEG_END) seg_end<FIELD, RECORD>(s, S, A);
+        if (__ballot(s.state != ST_DONE) == 0ull) break;
+        if (__ballot(s.state == ST_MARCH) == 0ull) continue;
+        for (int k = 0; k < A.spc; ++k) {
+            if (s.state == ST_MARCH) march_step<FIELD>(s, S);
+        }
+    }
+    // counters: wave reduction, one atomic per counter per wave
+#pragma unroll
+    for (int i = 0; i < C_N; ++i) {
+        unsigned long long v = s.c[i];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0 && v) atomicAdd(A.counters + i, v);
+    }
+}
+
+__global__ void k_fill_f32(float *p, size_t n, float v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long *p, int n) {
+    const int i = threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+template <int FIELD>
+__global__ void k_field_eval(DevScene S, int n, const float *pts, float scaling, float tau, float *out_n,
+                             float *out_g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float nn;
+    f3 g;
+    field_eval<FIELD>(S, mk3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]), scaling, tau, nn, g);
+    out_n[i] = nn;
+    out_g[3 * i] = g.x;
+    out_g[3 * i + 1] = g.y;
+    out_g[3 * i + 2] = g.z;
+}
+
+}  // namespace simpler
+
+// ======================================================================
+// Host side: the C ABI
+// ======================================================================
+using namespace simpler;
+
+struct simpler_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t cur = nullptr;
+    unsigned long long *d_scratch = nullptr;  // [0..7] work, [8..15] counters
+    int num_cu = 256;
+    int block = 256, spc = 32, bpc = 4;
+    std::string err;
+};
+
+struct simpler_scene {
+    simpler_ctx *ctx = nullptr;
+    simpler_scene_desc desc;
+    float *d_cdf = nullptr;
+    int ncdf = 0, tex_x = 1, tex_y = 1;
+    float4 *d_src = nullptr;
+    float *d_phase = nullptr;
+    int n_src = 0, n_coeff = 0;
+    float coeff_w[8] = {0};
+    std::vector<float> h_src;  // x y z a per source (host copy for bounds)
+    float *d_spline = nullptr;
+    int sN[3] = {0, 0, 0};
+    float sxmin[3] = {0, 0, 0}, sxmax[3] = {0, 0, 0};
+    float spline_min = 0.f;
+};
+
+static int fail(simpler_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                    \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail((ctx), SIMPLER_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+static inline float norm3(const float *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+// The dead-path cutoff (see the kernel header): pathlength_range.y plus a
+// margin bounding how much the final camera step can shorten the optical
+// length of a splat that still lands on the sensor.  Infinite without a time
+// gate, with an active sensor lens, or when n > 0 is not guaranteed.
+static double dead_path_cut(const simpler_scene *sc) {
+    const simpler_scene_desc &d = sc->desc;
+    if (d.flags & SIMPLER_FLAG_NO_DEAD_CUT) return INFINITY;
+    if (d.pathlength_range[0] == -1 && d.pathlength_range[1] == -1) return INFINITY;
+    if (d.sensor_lens_active) return INFINITY;
+    double nlow = d.n_o;
+    if (d.field_kind == SIMPLER_FIELD_BESSEL) {
+        nlow = d.n_o - std::fabs((double)d.n_max * (double)(d.n_clip / d.n_max));
+        if (!(std::isfinite(nlow))) nlow = d.n_o - std::fabs((double)d.n_max);
+    } else if (d.field_kind == SIMPLER_FIELD_SOURCES) {
+        double w = 0, s = 0;
+        for (int c = 0; c < sc->n_coeff; c++) w += std::fabs((double)sc->coeff_w[c]);
+        for (int i = 0; i < sc->n_src; i++) {
+            double dist2 = 0;
+            for (int k = 0; k < 3; k++) {
+                const double x = sc->h_src[4 * i + k];
+                const double e = x < d.block_l[k] ? d.block_l[k] - x : (x > d.block_r[k] ? x - d.block_r[k] : 0.0);
+                dist2 += e * e;
+            }
+            if (dist2 <= 0) return INFINITY;
+            s += std::fabs((double)sc->h_src[4 * i + 3]) / std::sqrt(dist2);
+        }
+        nlow = d.n_o - w * s;
+    } else if (d.field_kind == SIMPLER_FIELD_SPLINE) {
+        nlow = std::min((double)sc->spline_min, (double)d.n_o);
+    }
+    if (!(nlow > 0)) return INFINITY;
+    const float ext[3] = {d.block_r[0] - d.block_l[0], d.block_r[1] - d.block_l[1], d.block_r[2] - d.block_l[2]};
+    const double margin =
+        2.0 * (std::sqrt((double)d.view_plane[0] * d.view_plane[0] + (double)d.view_plane[1] * d.view_plane[1]) +
+               norm3(ext) + norm3(d.view_origin)) +
+        (double)d.er_stepsize;
+    return (double)d.pathlength_range[1] + margin;
+}
+
+static f3 h3(const float *v) { return f3{v[0], v[1], v[2]}; }
+static f3 hnorm(f3 a) {
+    const float l = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    return f3{a.x / l, a.y / l, a.z / l};
+}
+static f3 hcross(f3 a, f3 b) { return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+static DevScene make_dev_scene(const simpler_scene *sc, int64_t n_total, uint64_t seed, bool record) {
+    const simpler_scene_desc &d = sc->desc;
+    DevScene S;
+    std::memset(&S, 0, sizeof(S));
+    S.blockL = h3(d.block_l);
+    S.blockR = h3(d.block_r);
+    S.ior = d.ior;
+    S.light_origin = h3(d.light_origin);
+    S.ct = std::cos(d.half_theta_limit);
+    S.plane[0] = d.light_plane[0];
+    S.plane[1] = d.light_plane[1];
+    S.pixsize[0] = d.light_plane[0] / (float)sc->tex_x;
+    S.pixsize[1] = d.light_plane[1] / (float)sc->tex_y;
+    S.tex_x = sc->tex_x;
+    S.ncdf = sc->ncdf;
+    S.cdf = sc->d_cdf;
+    S.elens_x = d.emitter_lens_origin[0];
+    S.elens_ap2 = d.emitter_lens_aperture * d.emitter_lens_aperture;
+    S.elens_f = d.emitter_lens_focal_length;
+    S.elens_active = d.emitter_lens_active;
+    S.cam_o = h3(d.view_origin);
+    S.cam_dir = hnorm(h3(d.view_dir));
+    S.cam_h = hnorm(h3(d.view_horizontal));
+    S.cam_v = hcross(S.cam_dir, S.cam_h);  // camera.cpp:36-39
+    S.view_plane[0] = d.view_plane[0];
+    S.view_plane[1] = d.view_plane[1];
+    S.pl0 = d.pathlength_range[0];
+    S.pl1 = d.pathlength_range[1];
+    S.nogate = (d.pathlength_range[0] == -1.f && d.pathlength_range[1] == -1.f);
+    S.bounce = d.use_bounce_decomposition;
+    S.slens_active = d.sensor_lens_active;
+    S.slens_ap2 = d.sensor_lens_aperture * d.sensor_lens_aperture;
+    S.slens_f = d.sensor_lens_focal_length;
+    S.n_o = d.n_o;
+    const float lambda = d.speed_u / d.f_u;  // rif_base.h:149-153
+    S.k_r = (2.0f * kPi) / lambda;
+    S.amp = d.n_max;
+    S.n_max_scaling = d.n_clip / d.n_max;  // rif_analytical.h:67
+    S.phi_min = d.phi_min;
+    S.phi_range = d.phi_max - d.phi_min;
+    S.time_coupling = d.time_coupling;
+    S.mode = d.mode;
+    S.general_axis = (d.flags & SIMPLER_FLAG_GENERAL_AXIS) ? 1 : 0;
+    S.uz = h3(d.axis_uz);
+    S.ux = h3(d.axis_ux);
+    S.uy = hcross(S.uz, S.ux);
+    S.pu = h3(d.p_u);
+    S.h0 = d.er_stepsize;
+    S.precision = d.precision;
+    S.egap = d.egap_end_loc_x;
+    S.sgap = d.sgap_begin_loc_x;
+    S.n_src = sc->n_src;
+    S.n_coeff = sc->n_coeff;
+    S.src = sc->d_src;
+    S.src_phase = sc->d_phase;
+    for (int i = 0; i < 8; ++i) S.coeff_w[i] = sc->coeff_w[i];
+    S.spline = sc->d_spline;
+    for (int i = 0; i < 3; ++i) {
+        S.sN[i] = sc->sN[i];
+        S.sxmin[i] = sc->sxmin[i];
+        S.sxres[i] = sc->sN[i] > 1 ? (float)(sc->sN[i] - 1) / (sc->sxmax[i] - sc->sxmin[i]) : 0.f;
+    }
+    // medium.h:24-38
+    const bool clear = d.sigma_t <= kEps;
+    S.sigma_t = clear ? 0.f : d.sigma_t;
+    S.mfp = clear ? 1.f : 1.0f / d.sigma_t;
+    S.albedo = clear ? 0.f : d.albedo;
+    S.g = d.g;
+    S.max_depth = d.max_depth;
+    S.max_pathlength = d.max_pathlength;
+    S.use_direct = d.use_direct;
+    S.use_nee = d.use_angular_sampling;
+    S.direct_leak = (d.flags & SIMPLER_FLAG_DIRECT_LEAK) ? 1 : 0;
+    S.use_fresnel = (d.flags & SIMPLER_FLAG_USE_FRESNEL) ? 1 : 0;
+    // scene.h:141-164 (fresnelTrans) and photon.h:76-80 (weight)
+    float fresnelTrans = 1.f;
+    if (S.use_fresnel && d.ior > 1.f) {
+        const f3 ld = h3(d.light_dir);
+        const float ry = ld.y / d.ior, rz = ld.z / d.ior;
+        float rx = std::sqrt(1.f - (ry * ry + rz * rz));
+        if (ld.x < 0) rx = -rx;
+        const float eta = d.ior;
+        const float Rs = (ld.x - eta * rx) / (ld.x + eta * rx);
+        const float Rp = (rx - eta * ld.x) / (rx + eta * ld.x);
+        const float F = (std::fabs(eta - 1.f) < kEps * std::max(1.f, std::fabs(eta))) ? 0.f : 0.5f * (Rs * Rs + Rp * Rp);
+        fresnelTrans = eta * eta * (1.f - F);
+    }
+    S.weight = d.Li * fresnelTrans / (float)n_total;
+    S.Lcut = record ? INFINITY : (float)dead_path_cut(sc);
+    S.seed_lo = (uint32_t)seed;
+    S.seed_hi = (uint32_t)(seed >> 32);
+    return S;
+}
+
+static int check_desc(simpler_ctx *ctx, const simpler_scene_desc *d) {
+    if (!d) return fail(ctx, SIMPLER_ERR_INVALID, "null scene descriptor");
+    if (d->field_kind < 0 || d->field_kind > 2) return fail(ctx, SIMPLER_ERR_INVALID, "field_kind out of range");
+    if (!(d->er_stepsize > 0.f)) return fail(ctx, SIMPLER_ERR_INVALID, "er_stepsize must be > 0");
+    if (d->precision < 0 || d->precision > 12) return fail(ctx, SIMPLER_ERR_INVALID, "precision out of range [0,12]");
+    if (d->mode < 0 || d->mode > 7) return fail(ctx, SIMPLER_ERR_INVALID, "Bessel mode out of range [0,7]");
+    if (!(d->f_u > 0.f) || !(d->speed_u > 0.f)) return fail(ctx, SIMPLER_ERR_INVALID, "f_u and speed_u must be > 0");
+    if (!(d->view_plane[0] > 0.f) || !(d->view_plane[1] > 0.f)) return fail(ctx, SIMPLER_ERR_INVALID, "view_plane must be > 0");
+    for (int i = 0; i < 3; ++i)
+        if (!(d->block_r[i] > d->block_l[i])) return fail(ctx, SIMPLER_ERR_INVALID, "block_r must exceed block_l");
+    return SIMPLER_OK;
+}
+
+template <bool RECORD>
+static void launch_render(simpler_ctx *ctx, int field, const DevScene &S, const RenderArgs &A) {
+    const int blocks = std::max(1, ctx->num_cu * ctx->bpc);
+    const dim3 grid(blocks), block(ctx->block);
+    switch (field) {
+        case SIMPLER_FIELD_BESSEL: hipLaunchKernelGGL((k_render<kBessel, RECORD>), grid, block, 0, ctx->cur, S, A); break;
+        case SIMPLER_FIELD_SOURCES: hipLaunchKernelGGL((k_render<kSources, RECORD>), grid, block, 0, ctx->cur, S, A); break;
+        default: hipLaunchKernelGGL((k_render<kSpline, RECORD>), grid, block, 0, ctx->cur, S, A); break;
+    }
+}
+
+static int scene_ready(simpler_scene *sc) {
+    if (!sc) return SIMPLER_ERR_INVALID;
+    if (sc->desc.field_kind == SIMPLER_FIELD_SOURCES && sc->n_src <= 0)
+        return fail(sc->ctx, SIMPLER_ERR_STATE, "point-source field without sources (simpler_scene_set_sources)");
+    if (sc->desc.field_kind == SIMPLER_FIELD_SPLINE && !sc->d_spline)
+        return fail(sc->ctx, SIMPLER_ERR_STATE, "spline field without coefficients (simpler_scene_set_spline)");
+    return SIMPLER_OK;
+}
+
+extern "C" {
+
+int simpler_abi_version(void) { return SIMPLER_ABI_VERSION; }
+
+void simpler_scene_desc_default(simpler_scene_desc *d) {
+    if (!d) return;
+    std::memset(d, 0, sizeof(*d));
+    const float L[3] = {-0.015f, -5.0f, -5.0f}, R[3] = {0.015f, 5.0f, 5.0f};
+    for (int i = 0; i < 3; ++i) {
+        d->block_l[i] = L[i];
+        d->block_r[i] = R[i];
+    }
+    d->ior = 1.333f;
+    d->light_origin[0] = R[0];
+    d->light_dir[0] = -1.f;
+    d->half_theta_limit = 12.8e-3f;
+    d->light_plane[0] = d->light_plane[1] = 0.002f;
+    d->Li = 75000.f;
+    d->emitter_lens_origin[0] = R[0];
+    d->emitter_lens_aperture = d->emitter_lens_focal_length = 0.015f;
+    d->view_origin[0] = L[0];
+    d->view_dir[0] = -1.f;
+    d->view_horizontal[1] = -1.f;
+    d->view_plane[0] = d->view_plane[1] = 0.002f;
+    d->pathlength_range[0] = 0.f;
+    d->pathlength_range[1] = 64.f;
+    d->use_bounce_decomposition = 1;
+    d->sensor_lens_origin[0] = L[0];
+    d->sensor_lens_aperture = d->sensor_lens_focal_length = 0.015f;
+    d->field_kind = SIMPLER_FIELD_BESSEL;
+    d->n_o = 1.3333f;
+    d->f_u = 5e6f;
+    d->speed_u = 1500.f;
+    d->n_max = d->n_clip = 1e-5f;
+    d->phi_min = d->phi_max = 1.5708f;
+    d->axis_uz[0] = -1.f;  // rif_base.h:146-148
+    d->axis_ux[2] = 1.f;
+    d->er_stepsize = 1e-3f;
+    d->precision = 4;
+    d->egap_end_loc_x = R[0];
+    d->sgap_begin_loc_x = L[0];
+    d->sigma_t = 0.f;
+    d->albedo = 1.f;
+    d->g = 0.f;
+    d->max_depth = -1;
+    d->max_pathlength = -1.f;
+    d->use_direct = 0;
+    d->use_angular_sampling = 1;
+    d->flags = SIMPLER_FLAGS_DEFAULT;
+}
+
+int simpler_ctx_create(int device, simpler_ctx **out) {
+    if (!out) return SIMPLER_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return SIMPLER_ERR_INVALID;
+    simpler_ctx *ctx = new simpler_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete ctx;
+        return SIMPLER_ERR_HIP;
+    }
+    (void)hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, device);
+    if (hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&ctx->d_scratch, 16 * sizeof(unsigned long long)) != hipSuccess) {
+        if (ctx->own) (void)hipStreamDestroy(ctx->own);
+        delete ctx;
+        return SIMPLER_ERR_HIP;
+    }
+    ctx->cur = ctx->own;
+    *out = ctx;
+    return SIMPLER_OK;
+}
+
+int simpler_ctx_destroy(simpler_ctx *ctx) {
+    if (!ctx) return SIMPLER_ERR_INVALID;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->cur);
+    (void)hipFree(ctx->d_scratch);
+    (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+    return SIMPLER_OK;
+}
+
+int simpler_ctx_set_stream(simpler_ctx *ctx, void *stream) {
+    if (!ctx) return SIMPLER_ERR_INVALID;
+    ctx->cur = stream ? (hipStream_t)stream : ctx->own;
+    return SIMPLER_OK;
+}
+
+void *simpler_ctx_get_stream(simpler_ctx *ctx) { return ctx ? (void *)ctx->cur : nullptr; }
+
+int simpler_ctx_synchronize(simpler_ctx *ctx) {
+    if (!ctx) return SIMPLER_ERR_INVALID;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
+    return SIMPLER_OK;
+}
+
+const char *simpler_last_error(simpler_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int simpler_ctx_set_tuning(simpler_ctx *ctx, int block_size, int steps_per_chunk, int blocks_per_cu) {
+    if (!ctx) return SIMPLER_ERR_INVALID;
+    if (block_size) {
+        if (block_size != 64 && block_size != 128 && block_size != 256)
+            return fail(ctx, SIMPLER_ERR_INVALID, "block_size must be 64, 128 or 256");
+        ctx->block = block_size;
+    }
+    if (steps_per_chunk) {
+        if (steps_per_chunk < 1 || steps_per_chunk > 4096)
+            return fail(ctx, SIMPLER_ERR_INVALID, "steps_per_chunk out of range [1,4096]");
+        ctx->spc = steps_per_chunk;
+    }
+    if (blocks_per_cu) {
+        if (blocks_per_cu < 1 || blocks_per_cu > 64) return fail(ctx, SIMPLER_ERR_INVALID, "blocks_per_cu out of range [1,64]");
+        ctx->bpc = blocks_per_cu;
+    }
+    return SIMPLER_OK;
+}
+
+int simpler_ctx_get_tuning(simpler_ctx *ctx, int *block_size, int *steps_per_chunk, int *blocks_per_cu) {
+    if (!ctx) return SIMPLER_ERR_INVALID;
+    if (block_size) *block_size = ctx->block;
+    if (steps_per_chunk) *steps_per_chunk = ctx->spc;
+    if (blocks_per_cu) *blocks_per_cu = ctx->bpc;
+    return SIMPLER_OK;
+}
+
+int simpler_scene_create(simpler_ctx *ctx, const simpler_scene_desc *desc, const float *texture, int tex_x, int tex_y,
+                         simpler_scene **out) {
+    if (!ctx || !out) return SIMPLER_ERR_INVALID;
+    *out = nullptr;
+    int rc = check_desc(ctx, desc);
+    if (rc) return rc;
+    static const float white = 1.0f;
+    if (!texture || tex_x <= 0 || tex_y <= 0) {
+        texture = &white;
+        tex_x = tex_y = 1;
+    }
+    // pmf.h:41-99 (FP32 running sum, normalised, last entry forced to 1)
+    const size_t n = (size_t)tex_x * tex_y;
+    std::vector<float> cdf(1, 0.0f);
+    cdf.reserve(n + 1);
+    for (size_t i = 0; i < n; ++i) {
+        if (!(texture[i] >= 0.f)) return fail(ctx, SIMPLER_ERR_INVALID, "texture values must be >= 0");
+        cdf.push_back(cdf.back() + texture[i]);
+    }
+    const float sum = cdf.back();
+    if (!(sum > 0.f)) return fail(ctx, SIMPLER_ERR_INVALID, "texture has no positive texel");
+    const float norm = 1.0f / sum;
+    for (size_t i = 1; i < cdf.size(); ++i) cdf[i] *= norm;
+    cdf.back() = 1.0f;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    simpler_scene *sc = new simpler_scene();
+    sc->ctx = ctx;
+    sc->desc = *desc;
+    sc->tex_x = tex_x;
+    sc->tex_y = tex_y;
+    sc->ncdf = (int)cdf.size();
+    if (hipMalloc(&sc->d_cdf, cdf.size() * sizeof(float)) != hipSuccess ||
+        hipMemcpy(sc->d_cdf, cdf.data(), cdf.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(sc->d_cdf);
+        delete sc;
+        return fail(ctx, SIMPLER_ERR_HIP, "texture CDF upload failed");
+    }
+    *out = sc;
+    return SIMPLER_OK;
+}
+
+int simpler_scene_destroy(simpler_scene *sc) {
+    if (!sc) return SIMPLER_ERR_INVALID;
+    (void)hipSetDevice(sc->ctx->device);
+    (void)hipStreamSynchronize(sc->ctx->cur);
+    (void)hipFree(sc->d_cdf);
+    (void)hipFree(sc->d_src);
+    (void)hipFree(sc->d_phase);
+    (void)hipFree(sc->d_spline);
+    delete sc;
+    return SIMPLER_OK;
+}
+
+int simpler_scene_update(simpler_scene *sc, const simpler_scene_desc *desc) {
+    if (!sc) return SIMPLER_ERR_INVALID;
+    const int rc = check_desc(sc->ctx, desc);
+    if (rc) return rc;
+    sc->desc = *desc;
+    return SIMPLER_OK;
+}
+
+int simpler_scene_get_desc(const simpler_scene *sc, simpler_scene_desc *out) {
+    if (!sc || !out) return SIMPLER_ERR_INVALID;
+    *out = sc->desc;
+    return SIMPLER_OK;
+}
+
+int simpler_scene_set_sources(simpler_scene *sc, int n_src, const float *pos, const float *amp, const float *phase,
+                              int n_coeff, const float *coeff_w) {
+    if (!sc) return SIMPLER_ERR_INVALID;
+    simpler_ctx *ctx = sc->ctx;
+    if (n_src <= 0 || !pos || !amp || !phase) return fail(ctx, SIMPLER_ERR_INVALID, "sources: need n_src > 0 and arrays");
+    if (n_coeff < 1 || n_coeff > 8 || !coeff_w) return fail(ctx, SIMPLER_ERR_INVALID, "sources: n_coeff must be in [1,8]");
+    std::vector<float> s4(4 * (size_t)n_src);
+    for (int i = 0; i < n_src; ++i) {
+        s4[4 * i] = pos[3 * i];
+        s4[4 * i + 1] = pos[3 * i + 1];
+        s4[4 * i + 2] = pos[3 * i + 2];
+        s4[4 * i + 3] = amp[i];
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
+    (void)hipFree(sc->d_src);
+    (void)hipFree(sc->d_phase);
+    sc->d_src = nullptr;
+    sc->d_phase = nullptr;
+    HIP_TRY(ctx, hipMalloc(&sc->d_src, s4.size() * sizeof(float)));
+    HIP_TRY(ctx, hipMalloc(&sc->d_phase, (size_t)n_src * sizeof(float)));
+    HIP_TRY(ctx, hipMemcpy(sc->d_src, s4.data(), s4.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(sc->d_phase, phase, (size_t)n_src * sizeof(float), hipMemcpyHostToDevice));
+    sc->h_src = s4;
+    sc->n_src = n_src;
+    sc->n_coeff = n_coeff;
+    for (int c = 0; c < 8; ++c) sc->coeff_w[c] = c < n_coeff ? coeff_w[c] : 0.f;
+    return SIMPLER_OK;
+}
+
+int simpler_scene_set_spline(simpler_scene *sc, const float *coeffs, const int32_t dims[3], const float xmin[3],
+                             const float xmax[3]) {
+    if (!sc) return SIMPLER_ERR_INVALID;
+    simpler_ctx *ctx = sc->ctx;
+    if (!coeffs || !dims || !xmin || !xmax) return fail(ctx, SIMPLER_ERR_INVALID, "spline: null argument");
+    for (int i = 0; i < 3; ++i) {
+        if (dims[i] < 2 || dims[i] > 4096) return fail(ctx, SIMPLER_ERR_INVALID, "spline: dims must be in [2,4096]");
+        if (!(xmax[i] > xmin[i])) return fail(ctx, SIMPLER_ERR_INVALID, "spline: xmax must exceed xmin");
+    }
+    const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
+    (void)hipFree(sc->d_spline);
+    sc->d_spline = nullptr;
+    HIP_TRY(ctx, hipMalloc(&sc->d_spline, n * sizeof(float)));
+    HIP_TRY(ctx, hipMemcpy(sc->d_spline, coeffs, n * sizeof(float), hipMemcpyHostToDevice));
+    float mn = coeffs[0];
+    for (size_t i = 1; i < n; ++i) mn = std::min(mn, coeffs[i]);
+    sc->spline_min = mn;
+    for (int i = 0; i < 3; ++i) {
+        sc->sN[i] = dims[i];
+        sc->sxmin[i] = xmin[i];
+        sc->sxmax[i] = xmax[i];
+    }
+    return SIMPLER_OK;
+}
+
+static int render_impl(simpler_scene *sc, int64_t n_photons, int64_t n_total, uint64_t seed, uint64_t offset,
+                       float *image, int X, int Y, int Z, int accumulate, bool record, float *rp, float *rd, float *rL,
+                       int32_t *rs) {
+    simpler_ctx *ctx = sc->ctx;
+    int rc = scene_ready(sc);
+    if (rc) return rc;
+    if (n_photons < 0) return fail(ctx, SIMPLER_ERR_INVALID, "n_photons must be >= 0");
+    if (!record) {
+        if (!image) return fail(ctx, SIMPLER_ERR_INVALID, "null image");
+        if (X <= 0 || Y <= 0 || Z <= 0) return fail(ctx, SIMPLER_ERR_INVALID, "image resolution must be positive");
+        if ((double)X * Y * Z > 2147483647.0) return fail(ctx, SIMPLER_ERR_INVALID, "image larger than 2^31 pixels");
+        if (n_total > 0 && n_total < n_photons) return fail(ctx, SIMPLER_ERR_INVALID, "n_total must be >= n_photons");
+    } else if (!rp || !rd || !rL || !rs) {
+        return fail(ctx, SIMPLER_ERR_INVALID, "null record array");
+    }
+    if (n_total <= 0) n_total = n_photons > 0 ? n_photons : 1;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const DevScene S = make_dev_scene(sc, n_total, seed, record);
+    RenderArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.image = image;
+    A.X = X;
+    A.Y = Y;
+    A.Z = Z;
+    A.offset = offset;
+    A.n = (uint64_t)n_photons;
+    A.work = ctx->d_scratch;
+    A.counters = ctx->d_scratch + 8;
+    A.rec_p = rp;
+    A.rec_d = rd;
+    A.rec_L = rL;
+    A.rec_s = rs;
+    A.spc = ctx->spc;
+    // zero work queue + counters (a fill kernel: safe under graph capture)
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, ctx->d_scratch, 16);
+    if (!record && !accumulate) {
+        const size_t npix = (size_t)X * Y * Z;
+        const int blocks = (int)std::min<size_t>((npix + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_fill_f32, dim3(blocks), dim3(256), 0, ctx->cur, image, npix, 0.0f);
+    }
+    if (n_photons > 0) {
+        if (record)
+            launch_render<true>(ctx, sc->desc.field_kind, S, A);
+        else
+            launch_render<false>(ctx, sc->desc.field_kind, S, A);
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    return SIMPLER_OK;
+}
+
+int simpler_read_stats(simpler_ctx *ctx, simpler_stats *st) {
+    if (!ctx || !st) return SIMPLER_ERR_INVALID;
+    unsigned long long c[8];
+    HIP_TRY(ctx, hipMemcpyAsync(c, ctx->d_scratch + 8, sizeof(c), hipMemcpyDeviceToHost, ctx->cur));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
+    st->photons = c[C_PHOT];
+    st->er_steps = c[C_STEPS];
+    st->scatters = c[C_SCAT];
+    st->nee_splats = c[C_NEE];
+    st->direct_splats = c[C_DIRECT];
+    st->face_events = c[C_FACE];
+    st->segments = c[C_SEG];
+    st->aborted = c[C_ABORT];
+    return SIMPLER_OK;
+}
+
+int simpler_render(simpler_scene *sc, int64_t n_photons, int64_t n_total, uint64_t seed, uint64_t offset,
+                   float *image_dev, int X, int Y, int Z, int accumulate, simpler_stats *stats) {
+    if (!sc) return SIMPLER_ERR_INVALID;
+    const int rc = render_impl(sc, n_photons, n_total, seed, offset, image_dev, X, Y, Z, accumulate, false, nullptr,
+                               nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    if (stats) return simpler_read_stats(sc->ctx, stats);
+    return SIMPLER_OK;
+}
+
+int simpler_render_host(simpler_scene *sc, int64_t n_photons, int64_t n_total, uint64_t seed, uint64_t offset,
+                        float *image_host, int X, int Y, int Z, int accumulate, simpler_stats *stats) {
+    if (!sc || !image_host) return SIMPLER_ERR_INVALID;
+    simpler_ctx *ctx = sc->ctx;
+    if (X <= 0 || Y <= 0 || Z <= 0) return fail(ctx, SIMPLER_ERR_INVALID, "image resolution must be positive");
+    const size_t bytes = (size_t)X * Y * Z * sizeof(float);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    float *d = nullptr;
+    HIP_TRY(ctx, hipMalloc(&d, bytes));
+    int rc = SIMPLER_OK;
+    if (accumulate && hipMemcpy(d, image_host, bytes, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(ctx, SIMPLER_ERR_HIP, "image upload failed");
+    if (!rc) rc = simpler_render(sc, n_photons, n_total, seed, offset, d, X, Y, Z, accumulate, stats);
+    if (!rc && hipMemcpyAsync(image_host, d, bytes, hipMemcpyDeviceToHost, ctx->cur) != hipSuccess)
+        rc = fail(ctx, SIMPLER_ERR_HIP, "image download failed");
+    if (!rc && hipStreamSynchronize(ctx->cur) != hipSuccess) rc = fail(ctx, SIMPLER_ERR_HIP, "render failed");
+    (void)hipStreamSynchronize(ctx->cur);
+    (void)hipFree(d);
+    return rc;
+}
+
+int simpler_trace_rays(simpler_scene *sc, int64_t n_rays, uint64_t seed, uint64_t offset, float *out_pos,
+                       float *out_dir, float *out_len, int32_t *out_status) {
+    if (!sc) return SIMPLER_ERR_INVALID;
+    return render_impl(sc, n_rays, n_rays, seed, offset, nullptr, 1, 1, 1, 0, true, out_pos, out_dir, out_len,
+                       out_status);
+}
+
+int simpler_field_eval(simpler_scene *sc, int n_pts, const float *pts_dev, float scaling, float tau, float *out_n_dev,
+                       float *out_g_dev) {
+    if (!sc) return SIMPLER_ERR_INVALID;
+    simpler_ctx *ctx = sc->ctx;
+    int rc = scene_ready(sc);
+    if (rc) return rc;
+    if (n_pts < 0 || (n_pts > 0 && (!pts_dev || !out_n_dev || !out_g_dev)))
+        return fail(ctx, SIMPLER_ERR_INVALID, "field_eval: bad arguments");
+    if (n_pts == 0) return SIMPLER_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const DevScene S = make_dev_scene(sc, 1, 0, true);
+    const dim3 grid((n_pts + 255) / 256), block(256);
+    switch (sc->desc.field_kind) {
+        case SIMPLER_FIELD_BESSEL:
+            hipLaunchKernelGGL(k_field_eval<kBessel>, grid, block, 0, ctx->cur, S, n_pts, pts_dev, scaling, tau, out_n_dev, out_g_dev);
+            break;
+        case SIMPLER_FIELD_SOURCES:
+            hipLaunchKernelGGL(k_field_eval<kSources>, grid, block, 0, ctx->cur, S, n_pts, pts_dev, scaling, tau, out_n_dev, out_g_dev);
+            break;
+        default:
+            hipLaunchKernelGGL(k_field_eval<kSpline>, grid, block, 0, ctx->cur, S, n_pts, pts_dev, scaling, tau, out_n_dev, out_g_dev);
+            break;
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    return SIMPLER_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,222 @@
+"""Python binding of the C ABI in include/simpler.h (libsimpler_hip.so).
+
+PyTorch is used only for device memory and the HIP stream; every photon is
+traced by the gfx950 kernels in csrc/render.hip.  There is no CPU fallback:
+constructing a Context without the built library or without a GPU raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import build as _build
+from ._abi import SceneDesc, Stats
+
+_LIB = None
+
+
+class SimplerError(RuntimeError):
+    pass
+
+
+def load_library():
+    """Load libsimpler_hip.so (built in-tree by simpler_amd.build)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = _build.hip_library_path()
+    if not os.path.exists(path):
+        raise SimplerError(f"{path} is missing: build it with `python -m simpler_amd.build`")
+    L = C.CDLL(path)
+    P = C.c_void_p
+    PP = C.POINTER(C.c_void_p)
+    fptr = C.POINTER(C.c_float)
+    L.simpler_abi_version.restype = C.c_int
+    L.simpler_scene_desc_default.argtypes = [C.POINTER(SceneDesc)]
+    L.simpler_ctx_create.argtypes = [C.c_int, PP]
+    L.simpler_ctx_destroy.argtypes = [P]
+    L.simpler_ctx_set_stream.argtypes = [P, P]
+    L.simpler_ctx_get_stream.argtypes = [P]
+    L.simpler_ctx_get_stream.restype = P
+    L.simpler_ctx_synchronize.argtypes = [P]
+    L.simpler_last_error.argtypes = [P]
+    L.simpler_last_error.restype = C.c_char_p
+    L.simpler_ctx_set_tuning.argtypes = [P, C.c_int, C.c_int, C.c_int]
+    L.simpler_ctx_get_tuning.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.simpler_scene_create.argtypes = [P, C.POINTER(SceneDesc), fptr, C.c_int, C.c_int, PP]
+    L.simpler_scene_destroy.argtypes = [P]
+    L.simpler_scene_update.argtypes = [P, C.POINTER(SceneDesc)]
+    L.simpler_scene_get_desc.argtypes = [P, C.POINTER(SceneDesc)]
+    L.simpler_scene_set_sources.argtypes = [P, C.c_int, fptr, fptr, fptr, C.c_int, fptr]
+    L.simpler_scene_set_spline.argtypes = [P, fptr, C.POINTER(C.c_int32), fptr, fptr]
+    L.simpler_render.argtypes = [P, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, P, C.c_int, C.c_int, C.c_int,
+                                 C.c_int, C.POINTER(Stats)]
+    L.simpler_render_host.argtypes = [P, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, fptr, C.c_int, C.c_int,
+                                      C.c_int, C.c_int, C.POINTER(Stats)]
+    L.simpler_trace_rays.argtypes = [P, C.c_int64, C.c_uint64, C.c_uint64, P, P, P, P]
+    L.simpler_field_eval.argtypes = [P, C.c_int, P, C.c_float, C.c_float, P, P]
+    L.simpler_read_stats.argtypes = [P, C.POINTER(Stats)]
+    for name in ("simpler_ctx_create", "simpler_ctx_destroy", "simpler_ctx_set_stream", "simpler_ctx_synchronize",
+                 "simpler_ctx_set_tuning", "simpler_ctx_get_tuning", "simpler_scene_create", "simpler_scene_destroy",
+                 "simpler_scene_update", "simpler_scene_get_desc", "simpler_scene_set_sources",
+                 "simpler_scene_set_spline", "simpler_render", "simpler_render_host", "simpler_trace_rays",
+                 "simpler_field_eval", "simpler_read_stats"):
+        getattr(L, name).restype = C.c_int
+    _LIB = L
+    return L
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Context:
+    """One GPU + one HIP stream (by default torch's current stream, so torch
+    events and kernels order with the renders)."""
+
+    def __init__(self, device=0, use_torch_stream=True):
+        import torch
+        if not torch.cuda.is_available():
+            raise SimplerError("no GPU visible: the engine has no CPU fallback")
+        self.L = load_library()
+        self.device = device
+        torch.cuda.set_device(device)
+        h = C.c_void_p()
+        rc = self.L.simpler_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise SimplerError(f"simpler_ctx_create failed ({rc})")
+        self.h = h
+        if use_torch_stream:
+            self.set_stream(torch.cuda.current_stream(device).cuda_stream)
+
+    def check(self, rc):
+        if rc != 0:
+            msg = self.L.simpler_last_error(self.h).decode()
+            raise SimplerError(f"simpler call failed ({rc}): {msg}")
+
+    def set_stream(self, stream_ptr):
+        self.check(self.L.simpler_ctx_set_stream(self.h, C.c_void_p(stream_ptr)))
+
+    def synchronize(self):
+        self.check(self.L.simpler_ctx_synchronize(self.h))
+
+    def set_tuning(self, block_size=0, steps_per_chunk=0, blocks_per_cu=0):
+        self.check(self.L.simpler_ctx_set_tuning(self.h, block_size, steps_per_chunk, blocks_per_cu))
+
+    def get_tuning(self):
+        b, s, c = C.c_int(), C.c_int(), C.c_int()
+        self.check(self.L.simpler_ctx_get_tuning(self.h, C.byref(b), C.byref(s), C.byref(c)))
+        return dict(block_size=b.value, steps_per_chunk=s.value, blocks_per_cu=c.value)
+
+    def read_stats(self):
+        st = Stats()
+        self.check(self.L.simpler_read_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.simpler_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """A scene uploaded to the device (descriptor + texture CDF, optional
+    point sources or spline coefficients)."""
+
+    def __init__(self, ctx, desc, texture=None, sources=None, spline=None):
+        self.ctx = ctx
+        self.L = ctx.L
+        self.desc = desc.copy()
+        h = C.c_void_p()
+        if texture is None:
+            rc = self.L.simpler_scene_create(ctx.h, C.byref(self.desc), None, 0, 0, C.byref(h))
+        else:
+            tex = np.ascontiguousarray(texture, np.float32)
+            rc = self.L.simpler_scene_create(ctx.h, C.byref(self.desc), _fp(tex), tex.shape[1], tex.shape[0],
+                                             C.byref(h))
+        ctx.check(rc)
+        self.h = h
+        if sources is not None:
+            self.set_sources(**sources)
+        if spline is not None:
+            self.set_spline(*spline)
+
+    def set_sources(self, pos, amp, phase, coeff):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1)
+        amp = np.ascontiguousarray(amp, np.float32)
+        phase = np.ascontiguousarray(phase, np.float32)
+        coeff = np.ascontiguousarray(coeff, np.float32)
+        self.ctx.check(self.L.simpler_scene_set_sources(self.h, len(amp), _fp(pos), _fp(amp), _fp(phase), len(coeff),
+                                                        _fp(coeff)))
+
+    def set_spline(self, coeffs, dims, xmin, xmax):
+        c = np.ascontiguousarray(coeffs, np.float32).reshape(-1)
+        self.ctx.check(self.L.simpler_scene_set_spline(self.h, _fp(c), (C.c_int32 * 3)(*dims),
+                                                       (C.c_float * 3)(*xmin), (C.c_float * 3)(*xmax)))
+
+    def update(self, desc):
+        self.desc = desc.copy()
+        self.ctx.check(self.L.simpler_scene_update(self.h, C.byref(self.desc)))
+
+    def render(self, n, res, seed=1, offset=0, n_total=None, image=None, accumulate=False, stats=True):
+        """Render photons [offset, offset+n) of an n_total-photon render into
+        a float32 CUDA tensor of shape (Z, Y, X); returns (image, stats)."""
+        import torch
+        X, Y, Z = res
+        if image is None:
+            image = torch.empty((Z, Y, X), dtype=torch.float32, device=f"cuda:{self.ctx.device}")
+        if image.dtype != torch.float32 or not image.is_contiguous() or image.numel() != X * Y * Z:
+            raise SimplerError("image must be a contiguous float32 tensor of X*Y*Z elements")
+        st = Stats()
+        self.ctx.check(self.L.simpler_render(self.h, n, n_total or n, seed, offset, C.c_void_p(image.data_ptr()), X,
+                                             Y, Z, int(accumulate), C.byref(st) if stats else None))
+        return image, (st.as_dict() if stats else None)
+
+    def render_host(self, n, res, seed=1, offset=0, n_total=None):
+        X, Y, Z = res
+        img = np.zeros(X * Y * Z, np.float32)
+        st = Stats()
+        self.ctx.check(self.L.simpler_render_host(self.h, n, n_total or n, seed, offset, _fp(img), X, Y, Z, 0,
+                                                  C.byref(st)))
+        return img.reshape(Z, Y, X), st.as_dict()
+
+    def trace_rays(self, n, seed=1, offset=0):
+        """Per-ray direct trace: (exit position, unit exit direction, optical
+        length, status) as CUDA tensors."""
+        import torch
+        dev = f"cuda:{self.ctx.device}"
+        p = torch.empty((n, 3), dtype=torch.float32, device=dev)
+        d = torch.empty((n, 3), dtype=torch.float32, device=dev)
+        L = torch.empty((n,), dtype=torch.float32, device=dev)
+        s = torch.empty((n,), dtype=torch.int32, device=dev)
+        self.ctx.check(self.L.simpler_trace_rays(self.h, n, seed, offset, C.c_void_p(p.data_ptr()),
+                                                 C.c_void_p(d.data_ptr()), C.c_void_p(L.data_ptr()),
+                                                 C.c_void_p(s.data_ptr())))
+        return p, d, L, s
+
+    def field(self, pts, scaling=1.0, tau=0.0):
+        """n and grad n at points (CUDA tensor (N, 3) float32)."""
+        import torch
+        pts = pts.contiguous().to(torch.float32)
+        n = torch.empty((pts.shape[0],), dtype=torch.float32, device=pts.device)
+        g = torch.empty((pts.shape[0], 3), dtype=torch.float32, device=pts.device)
+        self.ctx.check(self.L.simpler_field_eval(self.h, pts.shape[0], C.c_void_p(pts.data_ptr()), scaling, tau,
+                                                 C.c_void_p(n.data_ptr()), C.c_void_p(g.data_ptr())))
+        return n, g
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.simpler_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

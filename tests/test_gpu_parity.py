@@ -1,0 +1,196 @@
+"""Parity of the gfx950 engine (through the C ABI) with the CPU oracle.
+
+* field:   n and grad n at random points vs the FP64 oracle (all field kinds);
+* per ray: transparent-medium exit position / direction / optical length vs
+           the FP64 oracle, each within 1e-5 relative;
+* image:   every bin within 3 sigma of the FP32 oracle render on the same
+           inputs (sigma from the spread over photon batches), total
+           throughput within 3 sigma, same event counts; an independent-seed
+           comparison for the statistics themselves;
+* properties at full size: chunked renders add up, renders are
+           deterministic, the dead-path cutoff leaves the image unchanged.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import gpu_threads
+from oracle import oracle as O
+from simpler_amd import engine, scenes
+from simpler_amd._abi import FLAG_GENERAL_AXIS, FLAG_NO_DEAD_CUT
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def _pts(n=2000, seed=0, half=0.005):
+    return np.random.default_rng(seed).uniform(-half, half, (n, 3))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("axis,general", [("z", True), ("x", True), ("x", False)])
+def test_bessel_field_matches_oracle(gpu_ctx, mode, axis, general):
+    d, tex, _ = scenes.bessel_cube(step=5e-6, n_max=2e-5, mode=mode, axis=axis)
+    d.flags = (d.flags | FLAG_GENERAL_AXIS) if general else (d.flags & ~FLAG_GENERAL_AXIS)
+    pts = _pts(seed=mode)
+    n_o, g_o = O.OracleScene(d, tex).field(pts, scaling=0.7)
+    n_g, g_g = engine.Scene(gpu_ctx, d, tex).field(torch.tensor(pts, dtype=torch.float32, device="cuda"), 0.7)
+    n_g, g_g = n_g.double().cpu().numpy(), g_g.double().cpu().numpy()
+    log(f"bessel m={mode} {axis} general={general}: dn {np.abs(n_g - n_o).max():.2e} "
+        f"dg {np.abs(g_g - g_o).max() / np.abs(g_o).max():.2e}")
+    assert np.abs(n_g - n_o).max() < 2e-7  # ~1 ulp of n = 1.333
+    assert np.abs(g_g - g_o).max() < 1e-5 * np.abs(g_o).max()
+
+
+def test_sources_field_matches_oracle(gpu_ctx):
+    d, tex, _, src = scenes.travelling_wave_cube()
+    pts = _pts(seed=7, half=0.0049)
+    n_o, g_o = O.OracleScene(d, tex, sources=src).field(pts, tau=1.3)
+    n_g, g_g = engine.Scene(gpu_ctx, d, tex, sources=src).field(
+        torch.tensor(pts, dtype=torch.float32, device="cuda"), 1.0, 1.3)
+    n_g, g_g = n_g.double().cpu().numpy(), g_g.double().cpu().numpy()
+    assert np.abs(n_g - n_o).max() < 2e-7
+    assert np.abs(g_g - g_o).max() < 1e-4 * np.abs(g_o).max()
+
+
+def test_spline_field_matches_oracle(gpu_ctx):
+    d, tex, _, (vals, dims, lo, hi) = scenes.spline_cube(n=24)
+    coeff = O.spline_build3d(vals.reshape(-1), dims)
+    spl = (coeff.astype(np.float32), dims, lo, hi)
+    pts = _pts(seed=9, half=0.0049)
+    n_o, g_o = O.OracleScene(d, tex, spline=spl).field(pts)
+    n_g, g_g = engine.Scene(gpu_ctx, d, tex, spline=spl).field(torch.tensor(pts, dtype=torch.float32, device="cuda"))
+    n_g, g_g = n_g.double().cpu().numpy(), g_g.double().cpu().numpy()
+    assert np.abs(n_g - n_o).max() < 4e-7
+    assert np.abs(g_g - g_o).max() < 1e-4 * np.abs(g_o).max()
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_transparent_per_ray_exit_matches_fp64(gpu_ctx, mode):
+    """configs: sigma_s = 0, per-ray exit p, d, L within 1e-5 relative."""
+    d, tex, _ = scenes.transparent_cube(mode=mode)
+    n = 20000
+    p, dr, L, s = engine.Scene(gpu_ctx, d, tex).trace_rays(n, seed=1)
+    p, dr, L, s = (t.double().cpu().numpy() for t in (p, dr, L, s))
+    p64, d64, L64, s64 = O.OracleScene(d, tex).trace_rays(n, seed=1, precision=64, threads=gpu_threads())
+    assert np.array_equal(s, s64) and (s == 1).all()
+    half = 0.005
+    ep = np.abs(p - p64).max() / half
+    ed = np.abs(dr - d64).max()
+    eL = (np.abs(L - L64) / L64).max()
+    log(f"mode {mode}: pos {ep:.2e} dir {ed:.2e} L {eL:.2e}")
+    assert ep < 1e-5 and ed < 1e-5 and eL < 1e-5
+
+
+def _batches(render, B):
+    return np.stack([render(b) for b in range(B)])
+
+
+def _bin_check(g, o):
+    """g, o: (B, Z, Y, X) batch images.  Every bin within 3 sigma (sigma of
+    the difference of the totals from the batch spread) and the totals within
+    3 sigma."""
+    B = g.shape[0]
+    Tg, To = g.sum(0), o.sum(0)
+    var = B * (g.var(0, ddof=1) + o.var(0, ddof=1))
+    sig = np.sqrt(var)
+    scale = max(Tg.max(), To.max())
+    bad = np.abs(Tg - To) > 3 * sig + 1e-6 * scale
+    tot_sig = math.sqrt(B * (g.sum((1, 2, 3)).var(ddof=1) + o.sum((1, 2, 3)).var(ddof=1)))
+    return int(bad.sum()), abs(Tg.sum() - To.sum()), tot_sig, int(((Tg > 0) | (To > 0)).sum())
+
+
+def test_render_every_bin_within_3_sigma_of_oracle(gpu_ctx):
+    d, tex, res = scenes.bessel_cube_small()
+    gs = engine.Scene(gpu_ctx, d, tex)
+    osc = O.OracleScene(d, tex)
+    B, n = 8, 48
+    N = B * n
+    g_st, o_st = [], []
+
+    def rg(b):
+        img, st = gs.render(n, res, seed=3, offset=b * n, n_total=N)
+        g_st.append(st)
+        return img.double().cpu().numpy()
+
+    def ro(b):
+        img, st = osc.render(n, res, seed=3, offset=b * n, n_total=N, precision=32, threads=gpu_threads(),
+                             cutoff=True)
+        o_st.append(st)
+        return img
+
+    g, o = _batches(rg, B), _batches(ro, B)
+    bad, dtot, tot_sig, nbins = _bin_check(g, o)
+    log(f"bins {nbins} bad {bad} total diff {dtot:.4g} sigma {tot_sig:.4g}")
+    assert bad == 0
+    assert dtot <= 3 * tot_sig
+    for k in ("photons", "scatters", "direct_splats"):
+        assert sum(s[k] for s in g_st) == sum(s[k] for s in o_st), k
+    ng, no = sum(s["nee_splats"] for s in g_st), sum(s["nee_splats"] for s in o_st)
+    assert abs(ng - no) <= 0.002 * no + 2
+
+
+def test_render_statistics_with_independent_seeds(gpu_ctx):
+    d, tex, res = scenes.bessel_cube(step=1e-5, n_max=1e-5, mode=2, res=(16, 16, 8))
+    gs = engine.Scene(gpu_ctx, d, tex)
+    osc = O.OracleScene(d, tex)
+    B, n = 8, 64
+    g = _batches(lambda b: gs.render(n, res, seed=101, offset=b * n, n_total=B * n)[0].double().cpu().numpy(), B)
+    o = _batches(lambda b: osc.render(n, res, seed=202, offset=b * n, n_total=B * n, precision=32,
+                                      threads=gpu_threads(), cutoff=True)[0], B)
+    bad, dtot, tot_sig, nbins = _bin_check(g, o)
+    log(f"independent seeds: bins {nbins} beyond 3 sigma {bad} total diff {dtot:.4g} sigma {tot_sig:.4g}")
+    assert dtot <= 3 * tot_sig
+    assert bad <= max(3, 0.03 * nbins)
+
+
+def test_chunked_render_adds_up_on_gpu(gpu_ctx):
+    d, tex, res = scenes.bessel_cube_full(mode=2)
+    gs = engine.Scene(gpu_ctx, d, tex)
+    N = 1 << 15
+    full, st = gs.render(N, res, seed=5, n_total=N)
+    full = full.double().cpu().numpy()
+    img = torch.zeros((res[2], res[1], res[0]), dtype=torch.float32, device="cuda")
+    gs.render(N // 4, res, seed=5, offset=0, n_total=N, image=img)
+    for k in range(1, 4):
+        gs.render(N // 4, res, seed=5, offset=k * (N // 4), n_total=N, image=img, accumulate=True)
+    chunks = img.double().cpu().numpy()
+    assert abs(chunks.sum() - full.sum()) < 1e-5 * full.sum()
+    assert np.abs(chunks - full).max() < 1e-4 * full.max()
+    again, st2 = gs.render(N, res, seed=5, n_total=N)
+    assert st == st2  # counters are exactly reproducible
+    assert np.abs(again.double().cpu().numpy() - full).max() < 1e-4 * full.max()
+
+
+def test_dead_path_cutoff_leaves_gpu_image_unchanged(gpu_ctx):
+    d, tex, res = scenes.bessel_cube_small()
+    n, seed = 48, 13
+    # the oracle runs the same photons first: a bounded full-work render
+    _, ost = O.OracleScene(d, tex).render(n, res, seed=seed, precision=32, threads=gpu_threads(), cutoff=False)
+    assert ost["er_steps"] < 5e8
+    cut, _ = engine.Scene(gpu_ctx, d, tex).render(n, res, seed=seed)
+    d2 = d.copy()
+    d2.flags |= FLAG_NO_DEAD_CUT
+    full, st = engine.Scene(gpu_ctx, d2, tex).render(n, res, seed=seed)
+    cut, full = cut.double().cpu().numpy(), full.double().cpu().numpy()
+    assert np.abs(cut - full).max() <= 1e-6 * full.max()
+    assert abs(st["er_steps"] - ost["er_steps"]) < 1e-3 * ost["er_steps"]
+
+
+def test_api_errors_are_reported(gpu_ctx):
+    d, tex, res = scenes.bessel_cube_small()
+    bad = d.copy()
+    bad.er_stepsize = -1.0
+    with pytest.raises(engine.SimplerError, match="er_stepsize"):
+        engine.Scene(gpu_ctx, bad, tex)
+    d2, tex2, res2, src = scenes.travelling_wave_cube()
+    sc = engine.Scene(gpu_ctx, d2, tex2)  # no sources attached
+    with pytest.raises(engine.SimplerError, match="without sources"):
+        sc.render(10, res2)
+    with pytest.raises(engine.SimplerError, match="n_total"):
+        engine.Scene(gpu_ctx, d, tex).render(100, res, n_total=10)
