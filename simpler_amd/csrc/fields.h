@@ -46,7 +46,8 @@ struct DevScene {
     const float4 *src;  // (x, y, z, amplitude)
     const float *src_phase;
     float coeff_w[8];
-    const float *spline;
+    const float *spline;  // coefficients minus spline_ref
+    float spline_ref;
     int sN[3];
     float sxmin[3], sxres[3];
     // medium
@@ -294,7 +295,9 @@ __device__ __forceinline__ void field_eval<kSpline>(const DevScene &S, f3 p, flo
         vy += sy0 * wz[c];
         vz += s0 * dwz[c];
     }
-    n = v;
+    // the weights sum to 1: n = ref + sum (c - ref) w keeps the rounding on
+    // the perturbation; the gradient weights sum to 0
+    n = S.spline_ref + v;
     g = mk3(vx * S.sxres[0], vy * S.sxres[1], vz * S.sxres[2]);
 }
 

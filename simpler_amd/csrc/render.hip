@@ -673,7 +673,8 @@ struct simpler_scene {
     int n_src = 0, n_coeff = 0;
     float coeff_w[8] = {0};
     std::vector<float> h_src;  // x y z a per source (host copy for bounds)
-    float *d_spline = nullptr;
+    float *d_spline = nullptr;  // coefficients minus spline_ref
+    float spline_ref = 0.f;
     int sN[3] = {0, 0, 0};
     float sxmin[3] = {0, 0, 0}, sxmax[3] = {0, 0, 0};
     float spline_min = 0.f;
@@ -796,6 +797,7 @@ static DevScene make_dev_scene(const simpler_scene *sc, int64_t n_total, uint64_
     S.src_phase = sc->d_phase;
     for (int i = 0; i < 8; ++i) S.coeff_w[i] = sc->coeff_w[i];
     S.spline = sc->d_spline;
+    S.spline_ref = sc->spline_ref;
     for (int i = 0; i < 3; ++i) {
         S.sN[i] = sc->sN[i];
         S.sxmin[i] = sc->sxmin[i];
@@ -1100,14 +1102,24 @@ int simpler_scene_set_spline(simpler_scene *sc, const float *coeffs, const int32
         if (!(xmax[i] > xmin[i])) return fail(ctx, SIMPLER_ERR_INVALID, "spline: xmax must exceed xmin");
     }
     const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+    // store c - ref with ref the coefficient mean (the device adds ref back)
+    double mean = 0.0;
+    float mn = coeffs[0];
+    for (size_t i = 0; i < n; ++i) {
+        if (!std::isfinite(coeffs[i])) return fail(ctx, SIMPLER_ERR_INVALID, "spline: non-finite coefficient");
+        mean += coeffs[i];
+        mn = std::min(mn, coeffs[i]);
+    }
+    const float ref = (float)(mean / (double)n);
+    std::vector<float> rel(n);
+    for (size_t i = 0; i < n; ++i) rel[i] = (float)((double)coeffs[i] - (double)ref);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
     (void)hipFree(sc->d_spline);
     sc->d_spline = nullptr;
     HIP_TRY(ctx, hipMalloc(&sc->d_spline, n * sizeof(float)));
-    HIP_TRY(ctx, hipMemcpy(sc->d_spline, coeffs, n * sizeof(float), hipMemcpyHostToDevice));
-    float mn = coeffs[0];
-    for (size_t i = 1; i < n; ++i) mn = std::min(mn, coeffs[i]);
+    HIP_TRY(ctx, hipMemcpy(sc->d_spline, rel.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    sc->spline_ref = ref;
     sc->spline_min = mn;
     for (int i = 0; i < 3; ++i) {
         sc->sN[i] = dims[i];
