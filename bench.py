@@ -110,6 +110,42 @@ def gpu_bench(args):
     return res
 
 
+def batch_bench(args):
+    """The batched workload: 256 transducer configurations (mixed modes,
+    Delta n, frequency, duty window, sigma_s, g; log-uniform photon budgets)
+    rendered by one persistent launch, eager and as a replayed HIP graph."""
+    import torch
+
+    from simpler_amd import engine, scenes
+
+    ctx = engine.Context(0)
+    d, tex, cfgs, res = scenes.batched_configs(n_cfg=args.batch_configs, seed=0, scale_budgets=args.batch_scale)
+    base = engine.Scene(ctx, d, tex)
+    batch = engine.Batch(base, cfgs, res)
+    batch.render()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, st = batch.render()
+    torch.cuda.synchronize()
+    eager = time.perf_counter() - t
+    batch.capture()
+    batch.replay()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.batch_replays):
+        batch.replay()
+    torch.cuda.synchronize()
+    graph = (time.perf_counter() - t) / args.batch_replays
+    eprint(f"batch: {batch.n_cfg} configurations, {batch.photons} photons, eager {eager:.2f}s, graph {graph:.2f}s")
+    out = dict(configurations=batch.n_cfg, photons=batch.photons, resolution=list(res),
+               photon_paths_per_s_eager=batch.photons / eager, photon_paths_per_s_graph=batch.photons / graph,
+               seconds_eager=eager, seconds_graph=graph, er_steps_per_photon=st["er_steps"] / st["photons"],
+               budget_scale=args.batch_scale)
+    batch.close()
+    ctx.close()
+    return out
+
+
 def cpu_baseline(args):
     from oracle import oracle as orc
     from simpler_amd import scenes
@@ -149,6 +185,11 @@ def main():
     ap.add_argument("--cpu-photons-cutoff", type=int, default=768)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-configs", type=int, default=256)
+    ap.add_argument("--batch-scale", type=float, default=0.02,
+                    help="photon budgets of the batched sweep times this factor (1.0: 5.5e8 photons)")
+    ap.add_argument("--batch-replays", type=int, default=2)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -188,6 +229,8 @@ def main():
                      "kernel": "k_render<Bessel> (persistent march + event kernel)",
                      "avg_launch_ms": g["kernel_ms"] / g["launches"]},
     }
+    if not args.no_batch:
+        out["batched"] = batch_bench(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

@@ -248,6 +248,42 @@ int simpler_field_eval(simpler_scene *scene, int n_pts, const float *pts_dev,
  * (synchronizes). */
 int simpler_read_stats(simpler_ctx *ctx, simpler_stats *stats_host);
 
+/* ---- batched launch: many transducer configurations with their own photon
+ *      budgets, one persistent kernel (reference: one renderImage call per
+ *      configuration, e.g. the sweeps of FUS_configurations.ipynb and
+ *      helperScripts/computePeakPhotonsPerVoltage.m:10-19).  The base scene
+ *      (a Bessel field) fixes geometry, source, sensor and flags; each
+ *      configuration overrides the fields below.  Configuration i renders
+ *      photons [0, n_photons) with its own seed and weight Li/n_photons into
+ *      image i of images_dev (n_cfg x z x y x x floats) -- exactly the image a
+ *      single-scene simpler_render of that configuration produces. ---------- */
+typedef struct simpler_batch_cfg {
+    float n_max;
+    float n_clip;
+    float f_u;
+    float phi_min;
+    float phi_max;
+    int32_t mode;
+    float sigma_t;
+    float albedo;
+    float g;
+    int32_t reserved;
+    int64_t n_photons;
+    uint64_t seed;
+} simpler_batch_cfg;
+
+typedef struct simpler_batch simpler_batch;
+
+int simpler_batch_create(simpler_scene *base, int n_cfg, const simpler_batch_cfg *cfgs_host, int x_res,
+                         int y_res, int z_res, simpler_batch **out);
+/* Asynchronous on the ctx stream; stats_host (may be NULL) synchronizes. */
+int simpler_batch_render(simpler_batch *batch, float *images_dev, int accumulate, simpler_stats *stats_host);
+/* Capture the zeroing + render of images_dev into a HIP graph (on the ctx's
+ * own stream); simpler_batch_graph_launch replays it on the ctx stream. */
+int simpler_batch_graph_capture(simpler_batch *batch, float *images_dev);
+int simpler_batch_graph_launch(simpler_batch *batch);
+int simpler_batch_destroy(simpler_batch *batch);
+
 /* Tunables of the persistent march kernel (picked by the autotuner):
  * block_size in {64,128,256,512}, steps_per_chunk >= 1 (march steps between
  * event phases), blocks_per_cu >= 1.  0 keeps the current value. */

@@ -114,3 +114,22 @@ class Stats(C.Structure):
 
     def as_dict(self):
         return {name: int(getattr(self, name)) for name, _ in self._fields_}
+
+
+class BatchCfg(C.Structure):
+    """``simpler_batch_cfg``: per-configuration overrides of a batched launch."""
+
+    _fields_ = [
+        ("n_max", C.c_float),
+        ("n_clip", C.c_float),
+        ("f_u", C.c_float),
+        ("phi_min", C.c_float),
+        ("phi_max", C.c_float),
+        ("mode", C.c_int32),
+        ("sigma_t", C.c_float),
+        ("albedo", C.c_float),
+        ("g", C.c_float),
+        ("reserved", C.c_int32),
+        ("n_photons", C.c_int64),
+        ("seed", C.c_uint64),
+    ]

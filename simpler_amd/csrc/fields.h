@@ -38,6 +38,8 @@ struct DevScene {
     // field
     float n_o, k_r, amp, n_max_scaling, phi_min, phi_range, time_coupling;
     int mode, general_axis;
+    const float4 *jtab;  // Bessel table (see bessel_tab)
+    int jtab_n;
     f3 uz, ux, uy, pu;
     float h0;
     int precision;
@@ -62,85 +64,51 @@ struct DevScene {
 
 enum FieldKind { kBessel = 0, kSources = 1, kSpline = 2 };
 
+// Field parameters that differ between the configurations of a batched
+// launch (uniform in a single-scene launch, per lane in a batched one).
+struct FieldP {
+    float amp, k_r;
+    int mode, jtab_n;
+    const float4 *jtab;
+};
+
+__device__ __forceinline__ FieldP field_params(const DevScene &S) {
+    FieldP F;
+    F.amp = S.amp;
+    F.k_r = S.k_r;
+    F.mode = S.mode;
+    F.jtab_n = S.jtab_n;
+    F.jtab = S.jtab;
+    return F;
+}
+
 // ---------------------------------------------------------------- Bessel
-// J_m(x) and J_{m+1}(x), 0 <= m <= 7, FP32, absolute error < ~1e-6 for the
-// arguments the engine meets (x = k r <= ~100).
-//   x < 3: power series J_n(x) = sum_j (-t)^j (x/2)^n / (j! (n+j)!), t = x^2/4
-//          (11 terms: last term < 2.25^11/(11!)^2 ~ 6e-12).
-//   x >= 3: Abramowitz & Stegun 9.4.3 / 9.4.6 for J0, J1 (rational
-//          asymptotic forms, |eps| < 1e-7), then upward recurrence
-//          J_{n+1} = (2n/x) J_n - J_{n-1}, stable for n < x.
-__device__ __forceinline__ float bessel_series(int n, float x) {
-    const float hx = 0.5f * x;
-    const float t = hx * hx;
-    float pw = 1.0f, fact = 1.0f;
-    for (int i = 1; i <= n; ++i) {
-        pw *= hx;
-        fact *= (float)i;
-    }
-    // Horner in t over j = 10..0 of (-1)^j / (j! (n+j)!) * n!
-    float s = 1.0f;
-#pragma unroll
-    for (int j = 10; j >= 1; --j) s = 1.0f - s * __fdividef(t, (float)(j * (n + j)));
-    return s * pw / fact;
-}
-
-__device__ __forceinline__ void bessel_j01_large(float x, float &j0, float &j1) {
-    const float y = 3.0f / x;
-    const float f0 = 0.79788456f +
-                     y * (-0.00000077f +
-                          y * (-0.00552740f +
-                               y * (-0.00009512f + y * (0.00137237f + y * (-0.00072805f + y * 0.00014476f)))));
-    const float t0 = -0.78539816f +
-                     y * (-0.04166397f +
-                          y * (-0.00003954f +
-                               y * (0.00262573f + y * (-0.00054125f + y * (-0.00029333f + y * 0.00013558f)))));
-    const float f1 = 0.79788456f +
-                     y * (0.00000156f +
-                          y * (0.01659667f +
-                               y * (0.00017105f + y * (-0.00249511f + y * (0.00113653f + y * -0.00020033f)))));
-    const float t1 = -2.35619449f +
-                     y * (0.12499612f +
-                          y * (0.00005650f +
-                               y * (-0.00637879f + y * (0.00074348f + y * (0.00079824f + y * -0.00029166f)))));
-    const float rs = rsqrtf(x);
-    // cos(x + t) with x up to ~100 rad: reduce x in revolutions first
-    const float inv2pi = 0.15915494309189535f;
-    const float xr = x * inv2pi;
-    const float xf = xr - rintf(xr);
-    const float a0 = xf * (2.0f * kPi) + t0;
-    const float a1 = xf * (2.0f * kPi) + t1;
-    j0 = rs * f0 * __cosf(a0);
-    j1 = rs * f1 * __cosf(a1);
-}
-
-__device__ __forceinline__ void bessel_pair(int m, float x, float &jm, float &jm1) {
-    if (x < 3.0f) {
-        jm = bessel_series(m, x);
-        jm1 = bessel_series(m + 1, x);
-    } else {
-        float a, b;
-        bessel_j01_large(x, a, b);  // J0, J1
-        const float ix = 1.0f / x;
-        for (int n = 1; n <= m; ++n) {
-            const float c = 2.0f * (float)n * ix * b - a;
-            a = b;
-            b = c;
-        }
-        jm = a;
-        jm1 = b;
-    }
+// J_m(x) and J_m'(x) from a per-scene table built on the host: on every unit
+// interval [i, i+1) of x = k r, degree-7 polynomials in u = x - (i + 1/2),
+// fitted in double at Chebyshev nodes to glibc jn (interpolation error
+// < 1e-9 for |J^(8)| <= 1).  Layout: 4 float4 per interval = 8 coefficients
+// of J_m, then 8 of J_m'.  One branch-free evaluation per step, no division,
+// no transcendental (the reference calls jn three times per evaluation,
+// scene.cpp:238-240).
+__device__ __forceinline__ void bessel_tab(const FieldP &F, float x, float &J, float &dJ) {
+    const int i = min((int)x, F.jtab_n - 1);
+    const float u = x - ((float)i + 0.5f);
+    const float4 *t = F.jtab + 4 * i;
+    const float4 a = t[0], b = t[1], c = t[2], d = t[3];
+    J = a.x + u * (a.y + u * (a.z + u * (a.w + u * (b.x + u * (b.y + u * (b.z + u * b.w))))));
+    dJ = c.x + u * (c.y + u * (c.z + u * (c.w + u * (d.x + u * (d.y + u * (d.z + u * d.w))))));
 }
 
 template <int FIELD>
-__device__ __forceinline__ void field_eval(const DevScene &S, f3 p, float scaling, float tau, float &n, f3 &g);
+__device__ __forceinline__ void field_eval(const DevScene &S, const FieldP &F, f3 p, float scaling, float tau,
+                                           float &n, f3 &g);
 
 // reference scene.cpp:205-257 (axis = +-x component form) or the same
 // gradient for a general axis: grad n = A s [J'(kr) k cos(m phi) rho/r
 //                                           - J m sin(m phi) (uz x rho)/r^2]
 template <>
-__device__ __forceinline__ void field_eval<kBessel>(const DevScene &S, f3 p, float scaling, float, float &n,
-                                                    f3 &g) {
+__device__ __forceinline__ void field_eval<kBessel>(const DevScene &S, const FieldP &F, f3 p, float scaling,
+                                                    float, float &n, f3 &g) {
     if (p.x > S.egap || p.x < S.sgap) {
         n = S.n_o;
         g = mk3(0.f, 0.f, 0.f);
@@ -149,31 +117,28 @@ __device__ __forceinline__ void field_eval<kBessel>(const DevScene &S, f3 p, flo
     const f3 q = p - S.pu;
     const float t = dot(q, S.uz);
     f3 rho = q - t * S.uz;
-    float r = len(rho);
-    const float cx = dot(rho, S.ux), cy = dot(rho, S.uy);
-    // cos(m phi), sin(m phi) from phi = atan2(cy, cx) by the angle-addition
-    // recurrence (atan2(0,0) = 0 at r = 0)
-    float c1 = 1.0f, s1 = 0.0f;
-    if (r > 0.0f) {
-        const float ir = 1.0f / r;
-        c1 = cx * ir;
-        s1 = cy * ir;
-    }
+    const float r2 = dot(rho, rho);
+    const bool nz = r2 > 0.f;
+    float ir = nz ? __builtin_amdgcn_rsqf(r2) : 0.f;
+    float r = r2 * ir;
+    // cos(m phi), sin(m phi) with phi = atan2(rho.uy, rho.ux) by the
+    // angle-addition recurrence (atan2(0,0) = 0 at r = 0)
+    const float c1 = nz ? dot(rho, S.ux) * ir : 1.0f;
+    const float s1 = nz ? dot(rho, S.uy) * ir : 0.0f;
     float cm = 1.0f, sm = 0.0f;
-    for (int k = 0; k < S.mode; ++k) {
+    for (int k = 0; k < F.mode; ++k) {
         const float cn = cm * c1 - sm * s1;
         sm = sm * c1 + cm * s1;
         cm = cn;
     }
-    const float A = S.amp * scaling;
-    float jm, jm1;
+    const float A = F.amp * scaling;
+    float J, dJ;
     const bool clamp = r < kEps;
     if (clamp) {
-        // the value uses the true radius; the gradient the clamped one
+        // the value uses the true radius, the gradient the clamped one
         // (reference scene.cpp:209-214 vs 229-234)
-        float dummy;
-        bessel_pair(S.mode, S.k_r * r, jm, dummy);
-        n = S.n_o + A * jm * cm;
+        bessel_tab(F, F.k_r * r, J, dJ);
+        n = S.n_o + A * J * cm;
         if (S.general_axis) {
             rho = kEps * (S.ux + S.uy);
         } else {
@@ -181,14 +146,12 @@ __device__ __forceinline__ void field_eval<kBessel>(const DevScene &S, f3 p, flo
             rho.z = kEps;
         }
         r = kEps;
+        ir = 1.0f / kEps;
     }
-    const float x = S.k_r * r;
-    bessel_pair(S.mode, x, jm, jm1);
-    if (!clamp) n = S.n_o + A * jm * cm;
-    const float dJ = (float)S.mode / x * jm - jm1;
-    const float invr = 1.0f / r;
-    const float cr = A * dJ * S.k_r * invr * cm;
-    const float ctan = A * jm * (float)S.mode * sm * invr * invr;
+    bessel_tab(F, F.k_r * r, J, dJ);
+    if (!clamp) n = S.n_o + A * J * cm;
+    const float cr = A * dJ * F.k_r * ir * cm;
+    const float ctan = A * J * (float)F.mode * sm * ir * ir;
     if (S.general_axis) {
         const f3 tp = cross(S.uz, rho);
         g = mk3(cr * rho.x - ctan * tp.x, cr * rho.y - ctan * tp.y, cr * rho.z - ctan * tp.z);
@@ -199,8 +162,8 @@ __device__ __forceinline__ void field_eval<kBessel>(const DevScene &S, f3 p, flo
 
 // reference scene.cpp:140-195, generalised (see simpler.h)
 template <>
-__device__ __forceinline__ void field_eval<kSources>(const DevScene &S, f3 p, float, float tau, float &n,
-                                                     f3 &g) {
+__device__ __forceinline__ void field_eval<kSources>(const DevScene &S, const FieldP &F, f3 p, float, float tau,
+                                                     float &n, f3 &g) {
     if (p.x > S.egap || p.x < S.sgap) {
         n = S.n_o;
         g = mk3(0.f, 0.f, 0.f);
@@ -218,7 +181,7 @@ __device__ __forceinline__ void field_eval<kSources>(const DevScene &S, f3 p, fl
         const float ph0 = S.src_phase[i] - tshift;
         float vs = 0.f, gs = 0.f;
         for (int c = 1; c <= S.n_coeff; ++c) {
-            const float krc = S.k_r * r * (float)c;
+            const float krc = F.k_r * r * (float)c;
             float sn, cs;
             __sincosf(krc + ph0, &sn, &cs);
             vs += S.coeff_w[c - 1] * cs;
@@ -255,7 +218,8 @@ __device__ __forceinline__ void bspline_w(float t, float w[4], float dw[4]) {
 }
 
 template <>
-__device__ __forceinline__ void field_eval<kSpline>(const DevScene &S, f3 p, float, float, float &n, f3 &g) {
+__device__ __forceinline__ void field_eval<kSpline>(const DevScene &S, const FieldP &, f3 p, float, float,
+                                                    float &n, f3 &g) {
     if (p.x > S.egap || p.x < S.sgap) {
         n = S.n_o;
         g = mk3(0.f, 0.f, 0.f);

@@ -38,7 +38,8 @@ namespace simpler {
 enum { PH_DIRECT = 0, PH_NEE = 1, PH_FLIGHT = 2 };
 enum { ST_MARCH = 0, ST_SEG_END = 1, ST_NEED = 2, ST_INIT = 3, ST_DONE = 4 };
 enum { C_PHOT = 0, C_STEPS, C_SCAT, C_NEE, C_DIRECT, C_FACE, C_SEG, C_ABORT, C_N };
-constexpr uint32_t kMaxFaceEvents = 1u << 20;  // per photon; see abort counter
+constexpr uint32_t kMaxFaceEvents = 1u << 20;  // per march; see the abort counter
+constexpr int kBlock = 256;
 
 struct RenderArgs {
     float *image;
@@ -49,58 +50,105 @@ struct RenderArgs {
     float *rec_p, *rec_d, *rec_L;
     int *rec_s;
     int spc;
+    // batched launch: per-configuration parameters and photon prefix sums
+    const struct Var *cfgs;
+    const unsigned long long *prefix;  // n_cfg + 1 entries
+    int n_cfg;
 };
+
+// Hot photon state: what the march loop reads and writes every step; kept in
+// VGPRs.
+struct Hot {
+    f3 p, pl, v, g0;  // position (+ compensation), v = n d, grad n at p
+    float n0, L, Ll;  // n at p, optical length (+ compensation)
+    float travelled, seg_dist, h, scaling, tau;
+    int prec;
+    uint32_t steps_left, steps;
+    int state;
+    bool dead;
+};
+
+// Cold photon state: touched only by the event code; one LDS slot per lane,
+// which keeps the march loop's register footprint small.
+struct Cold {
+    uint32_t pid_lo, pid_hi, stream, draw, main_draw, faces;
+    int phase, depth, first_flight;
+    float acc, large, mL, total_dist;
+    f3 mp, mv;  // main-path position / direction saved across NEE and direct
+};
+
+// Per-configuration photon parameters: uniform in a single-scene launch, one
+// LDS slot per lane in a batched launch.
+struct Var {
+    FieldP f;
+    float n_max_scaling, phi_min, phi_range, sigma_t, mfp, albedo, g, weight, Lcut;
+    uint32_t seed_lo, seed_hi;
+    float *image;
+};
+
+__device__ __forceinline__ Var var_from(const DevScene &S, const RenderArgs &A) {
+    Var v;
+    v.f = field_params(S);
+    v.n_max_scaling = S.n_max_scaling;
+    v.phi_min = S.phi_min;
+    v.phi_range = S.phi_range;
+    v.sigma_t = S.sigma_t;
+    v.mfp = S.mfp;
+    v.albedo = S.albedo;
+    v.g = S.g;
+    v.weight = S.weight;
+    v.Lcut = S.Lcut;
+    v.seed_lo = S.seed_lo;
+    v.seed_hi = S.seed_hi;
+    v.image = A.image;
+    return v;
+}
 
 struct Lane {
-    uint32_t pid_lo, pid_hi, stream, draw, main_draw;
-    f3 p, pl, v, g0;
-    float n0, L, Ll;
-    float travelled, seg_dist, h;
-    int prec;
-    uint32_t steps_left;
-    int state, phase;
-    bool dead, first_flight;
-    float acc, large;
-    f3 mp, mv;
-    float mL;
-    int depth;
-    float total_dist, scaling, tau;
-    uint32_t faces;
-    uint32_t c[C_N];
+    Hot &h;
+    Cold &c;
+    Var &v;
+    const DevScene &S;
+    const RenderArgs &A;
+    uint32_t *cnt;  // block-level event counters in LDS
 };
 
-__device__ __forceinline__ float draw_u(Lane &s, const DevScene &S) {
+__device__ __forceinline__ void count(Lane &L, int k) { atomicAdd(L.cnt + k, 1u); }
+
+__device__ __forceinline__ float draw_u(Lane &L) {
+    Cold &c = L.c;
     uint32_t o[4];
-    philox4x32_10(s.pid_lo, s.pid_hi, s.draw >> 2, s.stream, S.seed_lo, S.seed_hi, o);
-    const uint32_t k = s.draw & 3u;
+    philox4x32_10(c.pid_lo, c.pid_hi, c.draw >> 2, c.stream, L.v.seed_lo, L.v.seed_hi, o);
+    const uint32_t k = c.draw & 3u;
     const uint32_t w = k == 0 ? o[0] : (k == 1 ? o[1] : (k == 2 ? o[2] : o[3]));
-    ++s.draw;
+    ++c.draw;
     return to_uniform(w);
 }
 
 template <int FIELD>
-__device__ __forceinline__ void refresh_field(Lane &s, const DevScene &S) {
-    field_eval<FIELD>(S, s.p, s.scaling, s.tau, s.n0, s.g0);
+__device__ __forceinline__ void refresh_field(Lane &L) {
+    field_eval<FIELD>(L.S, L.v.f, L.h.p, L.h.scaling, L.h.tau, L.h.n0, L.h.g0);
 }
 
-__device__ __forceinline__ void start_segment(Lane &s, const DevScene &S, float dist) {
-    s.travelled = 0.f;
-    s.seg_dist = dist;
-    s.h = S.h0;
-    s.prec = S.precision;
-    const float q = dist / S.h0 + 1.0f;  // traceTillBlock maxsteps (scene.cpp:391)
-    s.steps_left = q >= 4294967040.0f ? 0xFFFFFFFFu : (uint32_t)q;
-    s.dead = false;
-    s.state = ST_MARCH;
-    ++s.c[C_SEG];
+__device__ __forceinline__ void start_segment(Lane &L, float dist) {
+    Hot &h = L.h;
+    h.travelled = 0.f;
+    h.seg_dist = dist;
+    h.h = L.S.h0;
+    h.prec = L.S.precision;
+    const float q = dist / L.S.h0 + 1.0f;  // traceTillBlock maxsteps (scene.cpp:391)
+    h.steps_left = q >= 4294967040.0f ? 0xFFFFFFFFu : (uint32_t)q;
+    h.dead = false;
+    h.state = ST_MARCH;
+    count(L, C_SEG);
 }
 
-__device__ __forceinline__ void end_segment(Lane &s) {
-    s.p = s.p + s.pl;
-    s.pl = mk3(0.f, 0.f, 0.f);
-    s.L += s.Ll;
-    s.Ll = 0.f;
-    s.state = ST_SEG_END;
+__device__ __forceinline__ void end_segment(Hot &h) {
+    h.p = h.p + h.pl;
+    h.pl = mk3(0.f, 0.f, 0.f);
+    h.L += h.Ll;
+    h.Ll = 0.f;
+    h.state = ST_SEG_END;
 }
 
 __device__ __forceinline__ bool inside_block(const DevScene &S, f3 p) {
@@ -113,47 +161,44 @@ __device__ __forceinline__ bool inside_block(const DevScene &S, f3 p) {
 // (h/2) grad n(p').  (n, grad n) at the start point is the previous step's
 // end-point evaluation.
 template <int FIELD>
-__device__ __forceinline__ void march_step(Lane &s, const DevScene &S) {
-    const float h = s.h;
-    const float hh = 0.5f * h;
-    const f3 vh = s.v + hh * s.g0;
-    const float hn = h / s.n0;
-    float px = s.p.x, py = s.p.y, pz = s.p.z;
-    float lx = s.pl.x, ly = s.pl.y, lz = s.pl.z;
+__device__ __forceinline__ void march_step(Hot &h, const DevScene &S, const FieldP &F, float Lcut) {
+    const float hs = h.h;
+    const float hh = 0.5f * hs;
+    const f3 vh = h.v + hh * h.g0;
+    const float hn = hs * __builtin_amdgcn_rcpf(h.n0);
+    float px = h.p.x, py = h.p.y, pz = h.p.z;
+    float lx = h.pl.x, ly = h.pl.y, lz = h.pl.z;
     comp_add(px, lx, hn * vh.x);
     comp_add(py, ly, hn * vh.y);
     comp_add(pz, lz, hn * vh.z);
     const f3 p1 = mk3(px, py, pz);
     float n1;
     f3 g1;
-    field_eval<FIELD>(S, p1, s.scaling, s.tau, n1, g1);
+    field_eval<FIELD>(S, F, p1, h.scaling, h.tau, n1, g1);
     const f3 v1 = vh + hh * g1;
-    ++s.c[C_STEPS];
-    if (!inside_block(S, p1) || (s.travelled + h) > s.seg_dist) {
-        if (--s.prec < 0) {  // stop past the face / distance (scene.cpp:405-407)
-            s.p = p1;
-            s.pl = mk3(lx, ly, lz);
-            s.v = v1;
-            s.n0 = n1;
-            s.g0 = g1;
-            end_segment(s);
-        } else {  // restore and refine: h/10, 10 more steps (scene.cpp:408-412)
-            s.h = h / 10.0f;
-            s.steps_left = 10;
-        }
+    ++h.steps;
+    const bool reject = !inside_block(S, p1) || (h.travelled + hs) > h.seg_dist;
+    const bool stop = reject && (--h.prec < 0);  // past the face / distance (scene.cpp:405-407)
+    if (!reject || stop) {
+        h.p = p1;
+        h.pl = mk3(lx, ly, lz);
+        h.v = v1;
+        h.n0 = n1;
+        h.g0 = g1;
+    }
+    if (stop) {
+        end_segment(h);
+    } else if (reject) {  // restore and refine: h/10, 10 more steps (scene.cpp:408-412)
+        h.h = hs / 10.0f;
+        h.steps_left = 10;
     } else {
-        s.p = p1;
-        s.pl = mk3(lx, ly, lz);
-        s.v = v1;
-        s.n0 = n1;
-        s.g0 = g1;
-        s.travelled += h;
-        comp_add(s.L, s.Ll, h * n1);  // optical length (scene.cpp:416)
-        if (s.L > S.Lcut) {
-            s.dead = true;
-            end_segment(s);
-        } else if (--s.steps_left == 0) {
-            end_segment(s);
+        h.travelled += hs;
+        comp_add(h.L, h.Ll, hs * n1);  // optical length (scene.cpp:416)
+        if (h.L > Lcut) {
+            h.dead = true;
+            end_segment(h);
+        } else if (--h.steps_left == 0) {
+            end_segment(h);
         }
     }
 }
@@ -190,9 +235,10 @@ __device__ __forceinline__ float fresnel_dielectric(float cosI, float cosT, floa
 }
 
 // bsdf.cpp:25-58 with util.h:59-88 (ior1 = 1, ior2 = scene ior)
-__device__ __forceinline__ f3 bsdf_sample(Lane &s, const DevScene &S, f3 in, f3 nrm) {
-    ++s.c[C_FACE];
-    ++s.faces;
+__device__ __forceinline__ f3 bsdf_sample(Lane &L, f3 in, f3 nrm) {
+    count(L, C_FACE);
+    ++L.c.faces;
+    const DevScene &S = L.S;
     if (fabsf(1.0f - S.ior) < kEps) return in;
     const float eta = dot(in, nrm) < -kEps ? S.ior : 1.0f / S.ior;
     const float dn = dot(in, nrm);
@@ -204,13 +250,13 @@ __device__ __forceinline__ f3 bsdf_sample(Lane &s, const DevScene &S, f3 in, f3 
     const float t = 1.0f - dot(pp, pp);
     const f3 outT = pp + (t > 0.f ? sqrtf(t) : 0.f) * q;
     const float F = fresnel_dielectric(fabsf(dot(nrm, in)), fabsf(dot(nrm, outT)), eta);
-    return draw_u(s, S) < F ? outR : outT;
+    return draw_u(L) < F ? outR : outT;
 }
 
 // phase.cpp:39-76 and util.h:17-27
-__device__ __forceinline__ f3 hg_sample(Lane &s, const DevScene &S, f3 in) {
-    const float g = S.g;
-    const float sx = draw_u(s, S), sy = draw_u(s, S);
+__device__ __forceinline__ f3 hg_sample(Lane &L, f3 in) {
+    const float g = L.v.g;
+    const float sx = draw_u(L), sy = draw_u(L);
     float cosT;
     if (fabsf(g) < kEps) {
         cosT = 1.f - 2.f * sx;
@@ -233,16 +279,15 @@ __device__ __forceinline__ f3 hg_sample(Lane &s, const DevScene &S, f3 in) {
     return (sinT * cp) * b + (sinT * sp) * c + cosT * in;
 }
 
-__device__ __forceinline__ float hg_f(const DevScene &S, f3 in, f3 out) {
-    const float g = S.g;
+__device__ __forceinline__ float hg_f(float g, f3 in, f3 out) {
     const float x = 1.f + g * g - 2.f * g * dot(in, out);
     return (1.0f / (4.0f * kPi)) * (1.f - g * g) / (x * sqrtf(x));
 }
 
 // scene.cpp:782-829 -> image.h:320-324, with the 4 bilinear taps as
 // device-wide float atomics
-__device__ __forceinline__ void splat(const DevScene &S, const RenderArgs &A, f3 p, float L, int depth,
-                                      float val) {
+__device__ __forceinline__ void splat(const DevScene &S, const RenderArgs &A, float *image, f3 p, float L,
+                                      int depth, float val) {
     float x = dot(S.cam_h, p) - S.cam_o.y;
     float y = dot(S.cam_v, p) - S.cam_o.z;
     if (!(S.nogate || (L > S.pl0 && L < S.pl1))) return;
@@ -264,7 +309,7 @@ __device__ __forceinline__ void splat(const DevScene &S, const RenderArgs &A, f3
     }
     if (iz < 0 || iz >= A.Z) return;
     const float fx = x - fx0, fy = y - fy0;
-    float *plane = A.image + (size_t)iz * A.X * A.Y;
+    float *plane = image + (size_t)iz * A.X * A.Y;
     const float w[4] = {val * (1.f - fx) * (1.f - fy), val * fx * (1.f - fy), val * (1.f - fx) * fy, val * fx * fy};
     const int tx[4] = {ix, ix + 1, ix, ix + 1};
     const int ty[4] = {iy, iy, iy + 1, iy + 1};
@@ -307,235 +352,249 @@ __device__ __forceinline__ bool exit_to_sensor(const DevScene &S, f3 &p, f3 d, f
     return true;
 }
 
-__device__ __forceinline__ void photon_end(Lane &s) { s.state = ST_NEED; }
+__device__ __forceinline__ void photon_end(Lane &L) { L.h.state = ST_NEED; }
 
-template <int FIELD>
-__device__ void flight_start(Lane &s, const DevScene &S, float dist) {
-    s.phase = PH_FLIGHT;
-    s.large = dist;
-    s.faces = 0;
-    start_segment(s, S, dist);
+__device__ __forceinline__ void flight_start(Lane &L, float dist) {
+    L.c.phase = PH_FLIGHT;
+    L.c.large = dist;
+    L.c.faces = 0;
+    start_segment(L, dist);
 }
 
 // photon.cpp:19-38
-template <int FIELD>
-__device__ void scatter_once(Lane &s, const DevScene &S) {
-    const float a = S.albedo;
-    if (!((a > 0.f) && ((a >= 1.f) || (draw_u(s, S) < a)))) {
-        photon_end(s);
+__device__ void scatter_once(Lane &L) {
+    const float a = L.v.albedo;
+    if (!((a > 0.f) && ((a >= 1.f) || (draw_u(L) < a)))) {
+        photon_end(L);
         return;
     }
-    const float mag = len(s.v);
-    const f3 d1 = hg_sample(s, S, s.v / mag);
-    ++s.c[C_SCAT];
-    s.v = mag * d1;
-    const float dist = -S.mfp * logf(draw_u(s, S));
-    flight_start<FIELD>(s, S, dist);
+    Hot &h = L.h;
+    const float mag = len(h.v);
+    const f3 d1 = hg_sample(L, h.v / mag);
+    count(L, C_SCAT);
+    h.v = mag * d1;
+    const float dist = -L.v.mfp * logf(draw_u(L));
+    flight_start(L, dist);
 }
 
 // photon.cpp:97-104: albedo test and first free flight
-template <int FIELD>
-__device__ void scatter_start(Lane &s, const DevScene &S) {
-    const float a = S.albedo;
-    if (!((a > 0.f) && ((a >= 1.f) || (draw_u(s, S) < a)))) {
-        photon_end(s);
+__device__ void scatter_start(Lane &L) {
+    const float a = L.v.albedo;
+    if (!((a > 0.f) && ((a >= 1.f) || (draw_u(L) < a)))) {
+        photon_end(L);
         return;
     }
-    const float dist = -S.mfp * logf(draw_u(s, S));
-    s.total_dist = dist;
-    s.first_flight = true;
-    flight_start<FIELD>(s, S, dist);
+    const float dist = -L.v.mfp * logf(draw_u(L));
+    L.c.total_dist = dist;
+    L.c.first_flight = 1;
+    flight_start(L, dist);
 }
 
-__device__ __forceinline__ void restore_main(Lane &s, float L) {
-    s.p = s.mp;
-    s.v = s.mv;
-    s.L = L;
-    s.stream = 0;
-    s.draw = s.main_draw;
+__device__ __forceinline__ void restore_main(Lane &L, float Lval) {
+    L.h.p = L.c.mp;
+    L.h.v = L.c.mv;
+    L.h.L = Lval;
+    L.c.stream = 0;
+    L.c.draw = L.c.main_draw;
 }
 
 // photon.cpp:111-116 loop head + scene.cpp:866-893 (NEE start)
-template <int FIELD>
-__device__ void vertex(Lane &s, const DevScene &S) {
-    if (!((S.max_depth < 0 || s.depth <= S.max_depth) && (S.max_pathlength < 0.f || s.total_dist <= S.max_pathlength))) {
-        photon_end(s);
+__device__ void vertex(Lane &L) {
+    const DevScene &S = L.S;
+    Hot &h = L.h;
+    Cold &c = L.c;
+    if (!((S.max_depth < 0 || c.depth <= S.max_depth) && (S.max_pathlength < 0.f || c.total_dist <= S.max_pathlength))) {
+        photon_end(L);
         return;
     }
-    if (S.use_nee && !((double)(s.p.x - S.cam_o.x) < 1e-4)) {
-        s.mp = s.p;
-        s.mv = s.v;
-        s.mL = s.L;
-        s.main_draw = s.draw;
-        s.stream = 1u + (uint32_t)s.depth;
-        s.draw = 0;
+    if (S.use_nee && !((double)(h.p.x - S.cam_o.x) < 1e-4)) {
+        c.mp = h.p;
+        c.mv = h.v;
+        c.mL = h.L;
+        c.main_draw = c.draw;
+        c.stream = 1u + (uint32_t)c.depth;
+        c.draw = 0;
         // scene.cpp:20-39 + warp.cpp:22-34: uniform hemisphere about -x
-        const float z = draw_u(s, S), s1 = draw_u(s, S);
+        const float z = draw_u(L), s1 = draw_u(L);
         const float tmp = sqrtf(1.f - z * z);
         float sa, ca;
         sincosf(2.0f * kPi * s1, &sa, &ca);
-        s.v = s.n0 * mk3(-z, ca * tmp, sa * tmp);
-        s.phase = PH_NEE;
-        s.acc = 0.f;
-        s.large = kLargeDist;
-        s.faces = 0;
-        start_segment(s, S, kLargeDist);
+        h.v = h.n0 * mk3(-z, ca * tmp, sa * tmp);
+        c.phase = PH_NEE;
+        c.acc = 0.f;
+        c.large = kLargeDist;
+        c.faces = 0;
+        start_segment(L, kLargeDist);
         return;
     }
-    scatter_once<FIELD>(s, S);
+    scatter_once(L);
 }
 
 template <int FIELD, bool RECORD>
-__device__ void direct_done(Lane &s, const DevScene &S, const RenderArgs &A, int res) {
+__device__ void direct_done(Lane &L, int res) {
+    const DevScene &S = L.S;
+    Hot &h = L.h;
+    Cold &c = L.c;
     if (RECORD) {
-        const size_t i = (size_t)((((uint64_t)s.pid_hi << 32) | s.pid_lo) - A.offset);
-        const f3 d = normalized(s.v);
-        A.rec_p[3 * i] = s.p.x;
-        A.rec_p[3 * i + 1] = s.p.y;
-        A.rec_p[3 * i + 2] = s.p.z;
+        const RenderArgs &A = L.A;
+        const size_t i = (size_t)((((uint64_t)c.pid_hi << 32) | c.pid_lo) - A.offset);
+        const f3 d = normalized(h.v);
+        A.rec_p[3 * i] = h.p.x;
+        A.rec_p[3 * i + 1] = h.p.y;
+        A.rec_p[3 * i + 2] = h.p.z;
         A.rec_d[3 * i] = d.x;
         A.rec_d[3 * i + 1] = d.y;
         A.rec_d[3 * i + 2] = d.z;
-        A.rec_L[i] = s.L;
+        A.rec_L[i] = h.L;
         A.rec_s[i] = res;
-        photon_end(s);
+        photon_end(L);
         return;
     }
     if (res == 1) {
-        f3 p = s.p, dts;
-        float L = s.L, fore, fw;
-        if (exit_to_sensor(S, p, s.v, s.n0, L, fore, fw, dts)) {
-            s.L = L;
-            ++s.c[C_DIRECT];
-            splat(S, A, p, L, 0, S.weight * expf(-S.sigma_t * s.acc) * fw);
+        f3 p = h.p, dts;
+        float Lv = h.L, fore, fw;
+        if (exit_to_sensor(S, p, h.v, h.n0, Lv, fore, fw, dts)) {
+            h.L = Lv;
+            count(L, C_DIRECT);
+            splat(S, L.A, L.v.image, p, Lv, 0, L.v.weight * expf(-L.v.sigma_t * c.acc) * fw);
         }
     }
     if (res == -2 && S.direct_leak) {  // the leaked length is already dead
-        photon_end(s);
+        photon_end(L);
         return;
     }
-    restore_main(s, S.direct_leak ? s.L : s.mL);  // photon.cpp:331-333
-    refresh_field<FIELD>(s, S);
-    scatter_start<FIELD>(s, S);
+    restore_main(L, S.direct_leak ? h.L : c.mL);  // photon.cpp:331-333
+    refresh_field<FIELD>(L);
+    scatter_start(L);
 }
 
 template <int FIELD>
-__device__ void nee_done(Lane &s, const DevScene &S, const RenderArgs &A, int res) {
+__device__ void nee_done(Lane &L, int res) {
+    const DevScene &S = L.S;
+    Hot &h = L.h;
+    Cold &c = L.c;
     if (res == 1) {
-        f3 p = s.p, dts;
-        float L = s.L, fore, fw;
-        if (exit_to_sensor(S, p, s.v, s.n0, L, fore, fw, dts)) {
-            const f3 din = s.mv / len(s.mv);
-            const float val = S.weight * (2.f * kPi) * expf(-S.sigma_t * s.acc) * hg_f(S, din, dts) * fore * fw;
-            ++s.c[C_NEE];
-            splat(S, A, p, L, s.depth, val);
+        f3 p = h.p, dts;
+        float Lv = h.L, fore, fw;
+        if (exit_to_sensor(S, p, h.v, h.n0, Lv, fore, fw, dts)) {
+            const f3 din = c.mv / len(c.mv);
+            const float val =
+                L.v.weight * (2.f * kPi) * expf(-L.v.sigma_t * c.acc) * hg_f(L.v.g, din, dts) * fore * fw;
+            count(L, C_NEE);
+            splat(S, L.A, L.v.image, p, Lv, c.depth, val);
         }
     }
-    restore_main(s, s.mL);
-    refresh_field<FIELD>(s, S);
-    scatter_once<FIELD>(s, S);
+    restore_main(L, c.mL);
+    refresh_field<FIELD>(L);
+    scatter_once(L);
 }
 
 // End of a march segment: the face / sensor logic of movePhoton
 // (scene.cpp:719-779) and movePhotonTillSensor (scene.cpp:634-700).
 template <int FIELD, bool RECORD>
-__device__ void seg_end(Lane &s, const DevScene &S, const RenderArgs &A) {
-    if (s.faces > kMaxFaceEvents && !s.dead) {  // no progress: abandon the march
-        ++s.c[C_ABORT];
-        s.dead = true;
+__device__ void seg_end(Lane &L) {
+    const DevScene &S = L.S;
+    Hot &h = L.h;
+    Cold &c = L.c;
+    if (c.faces > kMaxFaceEvents && !h.dead) {  // no progress: abandon the march
+        count(L, C_ABORT);
+        h.dead = true;
     }
-    if (s.phase == PH_FLIGHT) {
-        if (s.dead) {
-            photon_end(s);
+    if (c.phase == PH_FLIGHT) {
+        if (h.dead) {
+            photon_end(L);
             return;
         }
-        s.large -= s.travelled;
-        if (s.large > kEps) {
-            const f3 nrm = nearest_face(S, s.p);
-            const float mag = len(s.v);
-            const f3 d1 = bsdf_sample(s, S, s.v / mag, nrm);
+        c.large -= h.travelled;
+        if (c.large > kEps) {
+            const f3 nrm = nearest_face(S, h.p);
+            const float mag = len(h.v);
+            const f3 d1 = bsdf_sample(L, h.v / mag, nrm);
             if (dot(d1, nrm) < 0.f) {
-                s.v = d1 * mag;
-                start_segment(s, S, s.large);
+                h.v = d1 * mag;
+                start_segment(L, c.large);
             } else {
-                photon_end(s);
+                photon_end(L);
             }
             return;
         }
-        if (s.first_flight) {
-            s.depth = 1;
-            s.first_flight = false;
+        if (c.first_flight) {
+            c.depth = 1;
+            c.first_flight = 0;
         } else {
-            ++s.depth;
+            ++c.depth;
         }
-        vertex<FIELD>(s, S);
+        vertex(L);
         return;
     }
     int res;
-    if (s.dead) {
+    if (h.dead) {
         res = -2;
     } else {
-        s.acc += s.travelled;
-        s.large -= s.travelled;
-        if (s.large < 0.f) {
+        c.acc += h.travelled;
+        c.large -= h.travelled;
+        if (c.large < 0.f) {
             res = -1;
         } else {
-            const f3 nrm = nearest_face(S, s.p);
+            const f3 nrm = nearest_face(S, h.p);
             if (fabsf(S.cam_dir.x - nrm.x) < kEps && fabsf(S.cam_dir.y - nrm.y) < kEps &&
                 fabsf(S.cam_dir.z - nrm.z) < kEps) {
                 res = 1;
             } else {
-                const f3 d1 = bsdf_sample(s, S, s.v, nrm);  // unnormalised v (scene.cpp:686)
+                const f3 d1 = bsdf_sample(L, h.v, nrm);  // unnormalised v (scene.cpp:686)
                 if (dot(d1, nrm) < 0.f) {
-                    s.v = d1;
-                    start_segment(s, S, s.large);
+                    h.v = d1;
+                    start_segment(L, c.large);
                     return;
                 }
                 res = 0;
             }
         }
     }
-    if (s.phase == PH_DIRECT)
-        direct_done<FIELD, RECORD>(s, S, A, res);
+    if (c.phase == PH_DIRECT)
+        direct_done<FIELD, RECORD>(L, res);
     else
-        nee_done<FIELD>(s, S, A, res);
+        nee_done<FIELD>(L, res);
 }
 
 // pmf.h:109-121
 __device__ __forceinline__ int cdf_sample(const DevScene &S, float u) {
-    const float *c = S.cdf;
+    const float *cdf = S.cdf;
     int lo = 0, hi = S.ncdf;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (c[mid] < u)
+        if (cdf[mid] < u)
             lo = mid + 1;
         else
             hi = mid;
     }
     int index = min(S.ncdf - 2, max(0, lo - 1));
-    while (c[index + 1] - c[index] == 0.f && index < S.ncdf - 2) ++index;
+    while (cdf[index + 1] - cdf[index] == 0.f && index < S.ncdf - 2) ++index;
     return index;
 }
 
 // photon.cpp:312-333 up to the direct trace: genRay (scene.cpp:107-137,
 // lens.cpp:19-33), duty-cycle phase, v = n d.
 template <int FIELD, bool RECORD>
-__device__ void photon_init(Lane &s, const DevScene &S, const RenderArgs &A) {
-    ++s.c[C_PHOT];
-    s.stream = 0;
-    s.draw = 0;
-    s.faces = 0;
-    s.pl = mk3(0.f, 0.f, 0.f);
-    s.Ll = 0.f;
+__device__ void photon_init(Lane &L) {
+    const DevScene &S = L.S;
+    Hot &h = L.h;
+    Cold &c = L.c;
+    count(L, C_PHOT);
+    c.stream = 0;
+    c.draw = 0;
+    c.faces = 0;
+    h.pl = mk3(0.f, 0.f, 0.f);
+    h.Ll = 0.f;
     f3 pos = S.light_origin;
-    const int pixel = cdf_sample(S, draw_u(s, S));
+    const int pixel = cdf_sample(S, draw_u(L));
     const int py = pixel / S.tex_x;
     const int px = pixel - py * S.tex_x;
-    pos.y += -S.plane[0] / 2.0f + (float)px * S.pixsize[0] + draw_u(s, S) * S.pixsize[0];
-    pos.z += -S.plane[1] / 2.0f + (float)py * S.pixsize[1] + draw_u(s, S) * S.pixsize[1];
-    const float z = draw_u(s, S) * (1.f - S.ct) + S.ct;
+    pos.y += -S.plane[0] / 2.0f + (float)px * S.pixsize[0] + draw_u(L) * S.pixsize[0];
+    pos.z += -S.plane[1] / 2.0f + (float)py * S.pixsize[1] + draw_u(L) * S.pixsize[1];
+    const float z = draw_u(L) * (1.f - S.ct) + S.ct;
     const float zt = sqrtf(1.f - z * z);
-    const float phi = draw_u(s, S) * 2.f * kPi;
+    const float phi = draw_u(L) * 2.f * kPi;
     float sp, cp;
     sincosf(phi, &sp, &cp);
     f3 dir = mk3(-z, zt * cp, zt * sp);
@@ -545,83 +604,110 @@ __device__ void photon_init(Lane &s, const DevScene &S, const RenderArgs &A) {
     total += dl;
     if (S.elens_active && !lens_deflect(S.elens_ap2, S.elens_f, pos, dir, total)) {
         if (RECORD) {
-            s.p = pos;
-            s.v = dir;
-            s.L = total;
-            s.mL = total;
-            direct_done<FIELD, RECORD>(s, S, A, -4);
+            h.p = pos;
+            h.v = dir;
+            h.L = total;
+            c.mL = total;
+            direct_done<FIELD, RECORD>(L, -4);
         } else {
-            photon_end(s);
+            photon_end(L);
         }
         return;
     }
-    s.tau = S.phi_min + S.phi_range * draw_u(s, S);
-    s.scaling = fmaxf(fminf(sinf(s.tau), S.n_max_scaling), -S.n_max_scaling);
-    s.p = pos;
-    refresh_field<FIELD>(s, S);
-    s.v = s.n0 * dir;
-    s.L = total;
+    h.tau = L.v.phi_min + L.v.phi_range * draw_u(L);
+    h.scaling = fmaxf(fminf(sinf(h.tau), L.v.n_max_scaling), -L.v.n_max_scaling);
+    h.p = pos;
+    refresh_field<FIELD>(L);
+    h.v = h.n0 * dir;
+    h.L = total;
     if (S.use_direct || RECORD) {
-        s.mp = s.p;
-        s.mv = s.v;
-        s.mL = s.L;
-        s.main_draw = s.draw;
-        s.stream = 1;
-        s.draw = 0;
-        s.phase = PH_DIRECT;
-        s.acc = 0.f;
-        s.large = kLargeDist;
-        s.faces = 0;
-        start_segment(s, S, kLargeDist);
+        c.mp = h.p;
+        c.mv = h.v;
+        c.mL = h.L;
+        c.main_draw = c.draw;
+        c.stream = 1;
+        c.draw = 0;
+        c.phase = PH_DIRECT;
+        c.acc = 0.f;
+        c.large = kLargeDist;
+        c.faces = 0;
+        start_segment(L, kLargeDist);
     } else {
-        scatter_start<FIELD>(s, S);
+        scatter_start(L);
     }
 }
 
-template <int FIELD, bool RECORD>
-__global__ void __launch_bounds__(256) k_render(DevScene S, RenderArgs A) {
-    Lane s;
-#pragma unroll
-    for (int i = 0; i < C_N; ++i) s.c[i] = 0;
-    s.state = ST_NEED;
-    s.dead = false;
+// The persistent kernel.  BATCH: photons of many configurations (shared
+// geometry, per-configuration field / medium / seed / image) in one launch;
+// the global photon counter runs over the concatenated photon budgets and
+// each lane maps its index to (configuration, photon) by binary search.
+template <int FIELD, bool RECORD, bool BATCH>
+__global__ void __launch_bounds__(kBlock) k_render(DevScene S, RenderArgs A) {
+    __shared__ Cold cold[kBlock];
+    __shared__ uint32_t cnt[C_N];
+    __shared__ Var vars[BATCH ? kBlock : 1];
+    if (threadIdx.x < C_N) cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    Var vsingle = var_from(S, A);
+    Var &v = BATCH ? vars[threadIdx.x] : vsingle;
+    Hot h;
+    h.steps = 0;
+    h.state = ST_NEED;
+    h.dead = false;
+    Lane L{h, cold[threadIdx.x], v, S, A, cnt};
     const int lane = __lane_id();
     while (true) {
         // fetch: one atomic per wave for all lanes that need a photon
-        const unsigned long long need = __ballot(s.state == ST_NEED);
+        const unsigned long long need = __ballot(h.state == ST_NEED);
         if (need) {
             const int leader = __ffsll((long long)need) - 1;
             unsigned long long base = 0;
             if (lane == leader) base = atomicAdd(A.work, (unsigned long long)__popcll(need));
             base = __shfl(base, leader);
-            if (s.state == ST_NEED) {
+            if (h.state == ST_NEED) {
                 const unsigned long long rank = __popcll(need & ((1ull << lane) - 1ull));
                 const unsigned long long idx = base + rank;
                 if (idx < A.n) {
-                    const uint64_t pid = A.offset + idx;
-                    s.pid_lo = (uint32_t)pid;
-                    s.pid_hi = (uint32_t)(pid >> 32);
-                    s.state = ST_INIT;
+                    uint64_t pid = A.offset + idx;
+                    if (BATCH) {
+                        int lo = 0, hi = A.n_cfg;  // largest c with prefix[c] <= idx
+                        while (hi - lo > 1) {
+                            const int mid = (lo + hi) >> 1;
+                            if (A.prefix[mid] <= idx)
+                                lo = mid;
+                            else
+                                hi = mid;
+                        }
+                        pid = idx - A.prefix[lo];
+                        v = A.cfgs[lo];
+                        v.image = A.image + (size_t)lo * A.X * A.Y * A.Z;
+                    }
+                    L.c.pid_lo = (uint32_t)pid;
+                    L.c.pid_hi = (uint32_t)(pid >> 32);
+                    h.state = ST_INIT;
                 } else {
-                    s.state = ST_DONE;
+                    h.state = ST_DONE;
                 }
             }
         }
-        if (s.state == ST_INIT) photon_init<FIELD, RECORD>(s, S, A);
-        if (s.state == ST_SEG_END) seg_end<FIELD, RECORD>(s, S, A);
-        if (__ballot(s.state != ST_DONE) == 0ull) break;
-        if (__ballot(s.state == ST_MARCH) == 0ull) continue;
+        if (h.state == ST_INIT) photon_init<FIELD, RECORD>(L);
+        if (h.state == ST_SEG_END) seg_end<FIELD, RECORD>(L);
+        if (__ballot(h.state != ST_DONE) == 0ull) break;
+        if (__ballot(h.state == ST_MARCH) == 0ull) continue;
+        const FieldP F = v.f;
+        const float Lcut = v.Lcut;
         for (int k = 0; k < A.spc; ++k) {
-            if (s.state == ST_MARCH) march_step<FIELD>(s, S);
+            if (h.state == ST_MARCH) march_step<FIELD>(h, S, F, Lcut);
         }
     }
-    // counters: wave reduction, one atomic per counter per wave
-#pragma unroll
-    for (int i = 0; i < C_N; ++i) {
-        unsigned long long v = s.c[i];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0 && v) atomicAdd(A.counters + i, v);
-    }
+    // counters: ER steps reduced over the wave, events from LDS, one atomic
+    // per counter per wave / block
+    unsigned long long st = h.steps;
+    for (int off = 32; off > 0; off >>= 1) st += __shfl_xor(st, off);
+    if (lane == 0 && st) atomicAdd(A.counters + C_STEPS, st);
+    __syncthreads();
+    if (threadIdx.x < C_N && threadIdx.x != C_STEPS && cnt[threadIdx.x])
+        atomicAdd(A.counters + threadIdx.x, (unsigned long long)cnt[threadIdx.x]);
 }
 
 __global__ void k_fill_f32(float *p, size_t n, float v) {
@@ -639,7 +725,7 @@ __global__ void k_field_eval(DevScene S, int n, const float *pts, float scaling,
     if (i >= n) return;
     float nn;
     f3 g;
-    field_eval<FIELD>(S, mk3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]), scaling, tau, nn, g);
+    field_eval<FIELD>(S, field_params(S), mk3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]), scaling, tau, nn, g);
     out_n[i] = nn;
     out_g[3 * i] = g.x;
     out_g[3 * i + 1] = g.y;
@@ -673,6 +759,8 @@ struct simpler_scene {
     int n_src = 0, n_coeff = 0;
     float coeff_w[8] = {0};
     std::vector<float> h_src;  // x y z a per source (host copy for bounds)
+    float4 *d_jtab = nullptr;  // Bessel J_m / J_m' table (see bessel_tab)
+    int jtab_n = 0;
     float *d_spline = nullptr;  // coefficients minus spline_ref
     float spline_ref = 0.f;
     int sN[3] = {0, 0, 0};
@@ -783,6 +871,8 @@ static DevScene make_dev_scene(const simpler_scene *sc, int64_t n_total, uint64_
     S.time_coupling = d.time_coupling;
     S.mode = d.mode;
     S.general_axis = (d.flags & SIMPLER_FLAG_GENERAL_AXIS) ? 1 : 0;
+    S.jtab = sc->d_jtab;
+    S.jtab_n = sc->jtab_n;
     S.uz = h3(d.axis_uz);
     S.ux = h3(d.axis_ux);
     S.uy = hcross(S.uz, S.ux);
@@ -835,6 +925,79 @@ static DevScene make_dev_scene(const simpler_scene *sc, int64_t n_total, uint64_
     return S;
 }
 
+// Bessel table for bessel_tab(): on each unit interval [i, i+1) of x = k r,
+// degree-7 polynomials in u = x - (i + 1/2) for J_m and J_m', interpolating
+// glibc jn at the 8 Chebyshev nodes of the interval (monomial coefficients
+// from the 8x8 Vandermonde system, solved in double).  The table covers
+// x up to k times the largest distance from the transducer axis to a block
+// corner, with margin.
+static int bessel_table_host(simpler_ctx *ctx, const simpler_scene_desc &d, std::vector<float> &tab, int &n_out) {
+    const float lambda = d.speed_u / d.f_u;  // rif_base.h:149-153, as the kernel's k
+    const double k = (double)((2.0f * kPi) / lambda);
+    const double uz[3] = {d.axis_uz[0], d.axis_uz[1], d.axis_uz[2]};
+    double rmax = 0;
+    for (int cnr = 0; cnr < 8; ++cnr) {
+        double q[3];
+        for (int i = 0; i < 3; ++i) q[i] = (double)((cnr >> i) & 1 ? d.block_r[i] : d.block_l[i]) - d.p_u[i];
+        const double t = q[0] * uz[0] + q[1] * uz[1] + q[2] * uz[2];
+        double r2 = 0;
+        for (int i = 0; i < 3; ++i) r2 += (q[i] - t * uz[i]) * (q[i] - t * uz[i]);
+        rmax = std::max(rmax, std::sqrt(r2));
+    }
+    const double xmax = k * rmax * 1.02 + 2.0;
+    if (!(xmax < 65536.0)) return fail(ctx, SIMPLER_ERR_INVALID, "Bessel field: k * r_max exceeds 65536");
+    const int n = (int)std::ceil(xmax) + 1;
+    const int m = d.mode;
+    tab.assign((size_t)16 * n, 0.0f);
+    double nodes[8];
+    for (int j = 0; j < 8; ++j) nodes[j] = 0.5 * std::cos(M_PI * (2 * j + 1) / 16.0);
+    for (int i = 0; i < n; ++i) {
+        const double c = i + 0.5;
+        for (int f = 0; f < 2; ++f) {
+            double V[8][9];
+            for (int j = 0; j < 8; ++j) {
+                const double x = c + nodes[j];
+                double pw = 1;
+                for (int e = 0; e < 8; ++e) {
+                    V[j][e] = pw;
+                    pw *= nodes[j];
+                }
+                V[j][8] = f == 0 ? jn(m, x) : (m == 0 ? -jn(1, x) : 0.5 * (jn(m - 1, x) - jn(m + 1, x)));
+            }
+            for (int col = 0; col < 8; ++col) {  // Gauss-Jordan with partial pivoting
+                int piv = col;
+                for (int r = col + 1; r < 8; ++r)
+                    if (std::fabs(V[r][col]) > std::fabs(V[piv][col])) piv = r;
+                for (int e = 0; e < 9; ++e) std::swap(V[col][e], V[piv][e]);
+                for (int r = 0; r < 8; ++r) {
+                    if (r == col) continue;
+                    const double fct = V[r][col] / V[col][col];
+                    for (int e = col; e < 9; ++e) V[r][e] -= fct * V[col][e];
+                }
+            }
+            for (int e = 0; e < 8; ++e) tab[(size_t)16 * i + 8 * f + e] = (float)(V[e][8] / V[e][e]);
+        }
+    }
+    n_out = n;
+    return SIMPLER_OK;
+}
+
+static int build_bessel_table(simpler_ctx *ctx, simpler_scene *sc) {
+    std::vector<float> tab;
+    int n = 0;
+    const int rc = bessel_table_host(ctx, sc->desc, tab, n);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
+    (void)hipFree(sc->d_jtab);
+    sc->d_jtab = nullptr;
+    sc->jtab_n = 0;
+    HIP_TRY(ctx, hipMalloc(&sc->d_jtab, tab.size() * sizeof(float)));
+    HIP_TRY(ctx, hipMemcpy(sc->d_jtab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+    sc->jtab_n = n;
+    return SIMPLER_OK;
+}
+
 static int check_desc(simpler_ctx *ctx, const simpler_scene_desc *d) {
     if (!d) return fail(ctx, SIMPLER_ERR_INVALID, "null scene descriptor");
     if (d->field_kind < 0 || d->field_kind > 2) return fail(ctx, SIMPLER_ERR_INVALID, "field_kind out of range");
@@ -853,14 +1016,16 @@ static void launch_render(simpler_ctx *ctx, int field, const DevScene &S, const 
     const int blocks = std::max(1, ctx->num_cu * ctx->bpc);
     const dim3 grid(blocks), block(ctx->block);
     switch (field) {
-        case SIMPLER_FIELD_BESSEL: hipLaunchKernelGGL((k_render<kBessel, RECORD>), grid, block, 0, ctx->cur, S, A); break;
-        case SIMPLER_FIELD_SOURCES: hipLaunchKernelGGL((k_render<kSources, RECORD>), grid, block, 0, ctx->cur, S, A); break;
-        default: hipLaunchKernelGGL((k_render<kSpline, RECORD>), grid, block, 0, ctx->cur, S, A); break;
+        case SIMPLER_FIELD_BESSEL: hipLaunchKernelGGL((k_render<kBessel, RECORD, false>), grid, block, 0, ctx->cur, S, A); break;
+        case SIMPLER_FIELD_SOURCES: hipLaunchKernelGGL((k_render<kSources, RECORD, false>), grid, block, 0, ctx->cur, S, A); break;
+        default: hipLaunchKernelGGL((k_render<kSpline, RECORD, false>), grid, block, 0, ctx->cur, S, A); break;
     }
 }
 
 static int scene_ready(simpler_scene *sc) {
     if (!sc) return SIMPLER_ERR_INVALID;
+    if (sc->desc.field_kind == SIMPLER_FIELD_BESSEL && !sc->d_jtab)
+        return fail(sc->ctx, SIMPLER_ERR_STATE, "Bessel table missing (scene not built as a Bessel field)");
     if (sc->desc.field_kind == SIMPLER_FIELD_SOURCES && sc->n_src <= 0)
         return fail(sc->ctx, SIMPLER_ERR_STATE, "point-source field without sources (simpler_scene_set_sources)");
     if (sc->desc.field_kind == SIMPLER_FIELD_SPLINE && !sc->d_spline)
@@ -1032,6 +1197,13 @@ int simpler_scene_create(simpler_ctx *ctx, const simpler_scene_desc *desc, const
         delete sc;
         return fail(ctx, SIMPLER_ERR_HIP, "texture CDF upload failed");
     }
+    if (desc->field_kind == SIMPLER_FIELD_BESSEL) {
+        const int rc2 = build_bessel_table(ctx, sc);
+        if (rc2) {
+            simpler_scene_destroy(sc);
+            return rc2;
+        }
+    }
     *out = sc;
     return SIMPLER_OK;
 }
@@ -1044,6 +1216,7 @@ int simpler_scene_destroy(simpler_scene *sc) {
     (void)hipFree(sc->d_src);
     (void)hipFree(sc->d_phase);
     (void)hipFree(sc->d_spline);
+    (void)hipFree(sc->d_jtab);
     delete sc;
     return SIMPLER_OK;
 }
@@ -1052,7 +1225,20 @@ int simpler_scene_update(simpler_scene *sc, const simpler_scene_desc *desc) {
     if (!sc) return SIMPLER_ERR_INVALID;
     const int rc = check_desc(sc->ctx, desc);
     if (rc) return rc;
+    const simpler_scene_desc old = sc->desc;
     sc->desc = *desc;
+    if (desc->field_kind == SIMPLER_FIELD_BESSEL &&
+        (old.field_kind != desc->field_kind || old.mode != desc->mode || old.f_u != desc->f_u ||
+         old.speed_u != desc->speed_u || std::memcmp(old.block_l, desc->block_l, sizeof(old.block_l)) ||
+         std::memcmp(old.block_r, desc->block_r, sizeof(old.block_r)) ||
+         std::memcmp(old.axis_uz, desc->axis_uz, sizeof(old.axis_uz)) ||
+         std::memcmp(old.p_u, desc->p_u, sizeof(old.p_u)) || !sc->d_jtab)) {
+        const int rc2 = build_bessel_table(sc->ctx, sc);
+        if (rc2) {
+            sc->desc = old;
+            return rc2;
+        }
+    }
     return SIMPLER_OK;
 }
 
@@ -1261,3 +1447,188 @@ int simpler_field_eval(simpler_scene *sc, int n_pts, const float *pts_dev, float
 }
 
 }  // extern "C"
+
+struct simpler_batch {
+    simpler_ctx *ctx = nullptr;
+    simpler_scene *base = nullptr;
+    int n_cfg = 0, X = 0, Y = 0, Z = 0;
+    uint64_t total = 0;
+    Var *d_cfgs = nullptr;
+    unsigned long long *d_prefix = nullptr;
+    float4 *d_jtab = nullptr;
+    DevScene S;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static int batch_launch(simpler_batch *b, float *images, int accumulate, hipStream_t stream) {
+    simpler_ctx *ctx = b->ctx;
+    RenderArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.image = images;
+    A.X = b->X;
+    A.Y = b->Y;
+    A.Z = b->Z;
+    A.offset = 0;
+    A.n = b->total;
+    A.work = ctx->d_scratch;
+    A.counters = ctx->d_scratch + 8;
+    A.spc = ctx->spc;
+    A.cfgs = b->d_cfgs;
+    A.prefix = b->d_prefix;
+    A.n_cfg = b->n_cfg;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, stream, ctx->d_scratch, 16);
+    if (!accumulate) {
+        const size_t npix = (size_t)b->n_cfg * b->X * b->Y * b->Z;
+        const int blocks = (int)std::min<size_t>((npix + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_fill_f32, dim3(blocks), dim3(256), 0, stream, images, npix, 0.0f);
+    }
+    if (b->total > 0) {
+        const int blocks = std::max(1, ctx->num_cu * ctx->bpc);
+        hipLaunchKernelGGL((k_render<kBessel, false, true>), dim3(blocks), dim3(ctx->block), 0, stream, b->S, A);
+    }
+    return SIMPLER_OK;
+}
+
+extern "C" {
+
+int simpler_batch_create(simpler_scene *base, int n_cfg, const simpler_batch_cfg *cfgs, int X, int Y, int Z,
+                         simpler_batch **out) {
+    if (!base || !out) return SIMPLER_ERR_INVALID;
+    *out = nullptr;
+    simpler_ctx *ctx = base->ctx;
+    if (base->desc.field_kind != SIMPLER_FIELD_BESSEL)
+        return fail(ctx, SIMPLER_ERR_INVALID, "batch: the base scene must be a Bessel field");
+    if (n_cfg <= 0 || !cfgs) return fail(ctx, SIMPLER_ERR_INVALID, "batch: need n_cfg > 0 configurations");
+    if (X <= 0 || Y <= 0 || Z <= 0 || (double)n_cfg * X * Y * Z > 4.0e9)
+        return fail(ctx, SIMPLER_ERR_INVALID, "batch: bad image resolution");
+    std::vector<Var> vars(n_cfg);
+    std::vector<unsigned long long> prefix(n_cfg + 1, 0ull);
+    std::vector<float> jtab_all;
+    std::vector<size_t> jtab_off(n_cfg);
+    for (int i = 0; i < n_cfg; ++i) {
+        const simpler_batch_cfg &c = cfgs[i];
+        if (c.n_photons < 0) return fail(ctx, SIMPLER_ERR_INVALID, "batch: n_photons must be >= 0");
+        simpler_scene tmp = *base;  // shallow: only descriptor values are used
+        tmp.desc.n_max = c.n_max;
+        tmp.desc.n_clip = c.n_clip;
+        tmp.desc.f_u = c.f_u;
+        tmp.desc.phi_min = c.phi_min;
+        tmp.desc.phi_max = c.phi_max;
+        tmp.desc.mode = c.mode;
+        tmp.desc.sigma_t = c.sigma_t;
+        tmp.desc.albedo = c.albedo;
+        tmp.desc.g = c.g;
+        int rc = check_desc(ctx, &tmp.desc);
+        if (rc) return rc;
+        std::vector<float> tab;
+        int nint = 0;
+        rc = bessel_table_host(ctx, tmp.desc, tab, nint);
+        if (rc) return rc;
+        jtab_off[i] = jtab_all.size();
+        jtab_all.insert(jtab_all.end(), tab.begin(), tab.end());
+        const DevScene Si = make_dev_scene(&tmp, c.n_photons > 0 ? c.n_photons : 1, c.seed, false);
+        Var &v = vars[i];
+        v.f.amp = Si.amp;
+        v.f.k_r = Si.k_r;
+        v.f.mode = Si.mode;
+        v.f.jtab_n = nint;
+        v.f.jtab = nullptr;  // patched below
+        v.n_max_scaling = Si.n_max_scaling;
+        v.phi_min = Si.phi_min;
+        v.phi_range = Si.phi_range;
+        v.sigma_t = Si.sigma_t;
+        v.mfp = Si.mfp;
+        v.albedo = Si.albedo;
+        v.g = Si.g;
+        v.weight = Si.weight;
+        v.Lcut = Si.Lcut;
+        v.seed_lo = Si.seed_lo;
+        v.seed_hi = Si.seed_hi;
+        v.image = nullptr;  // set in the kernel from the configuration index
+        prefix[i + 1] = prefix[i] + (unsigned long long)c.n_photons;
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    simpler_batch *b = new simpler_batch();
+    b->ctx = ctx;
+    b->base = base;
+    b->n_cfg = n_cfg;
+    b->X = X;
+    b->Y = Y;
+    b->Z = Z;
+    b->total = prefix[n_cfg];
+    if (hipMalloc(&b->d_jtab, jtab_all.size() * sizeof(float)) != hipSuccess ||
+        hipMalloc(&b->d_cfgs, vars.size() * sizeof(Var)) != hipSuccess ||
+        hipMalloc(&b->d_prefix, prefix.size() * sizeof(unsigned long long)) != hipSuccess) {
+        simpler_batch_destroy(b);
+        return fail(ctx, SIMPLER_ERR_NOMEM, "batch: device allocation failed");
+    }
+    for (int i = 0; i < n_cfg; ++i) vars[i].f.jtab = b->d_jtab + jtab_off[i] / 4;
+    if (hipMemcpy(b->d_jtab, jtab_all.data(), jtab_all.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_cfgs, vars.data(), vars.size() * sizeof(Var), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_prefix, prefix.data(), prefix.size() * sizeof(unsigned long long), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+        simpler_batch_destroy(b);
+        return fail(ctx, SIMPLER_ERR_HIP, "batch: upload failed");
+    }
+    b->S = make_dev_scene(base, 1, 0, false);
+    *out = b;
+    return SIMPLER_OK;
+}
+
+int simpler_batch_render(simpler_batch *b, float *images, int accumulate, simpler_stats *stats) {
+    if (!b) return SIMPLER_ERR_INVALID;
+    simpler_ctx *ctx = b->ctx;
+    if (!images) return fail(ctx, SIMPLER_ERR_INVALID, "batch: null images");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    batch_launch(b, images, accumulate, ctx->cur);
+    HIP_TRY(ctx, hipGetLastError());
+    if (stats) return simpler_read_stats(ctx, stats);
+    return SIMPLER_OK;
+}
+
+int simpler_batch_graph_capture(simpler_batch *b, float *images) {
+    if (!b) return SIMPLER_ERR_INVALID;
+    simpler_ctx *ctx = b->ctx;
+    if (!images) return fail(ctx, SIMPLER_ERR_INVALID, "batch: null images");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->own, hipStreamCaptureModeThreadLocal));
+    batch_launch(b, images, 0, ctx->own);
+    HIP_TRY(ctx, hipStreamEndCapture(ctx->own, &b->graph));
+    HIP_TRY(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return SIMPLER_OK;
+}
+
+int simpler_batch_graph_launch(simpler_batch *b) {
+    if (!b) return SIMPLER_ERR_INVALID;
+    simpler_ctx *ctx = b->ctx;
+    if (!b->exec) return fail(ctx, SIMPLER_ERR_STATE, "batch: no captured graph (simpler_batch_graph_capture)");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipGraphLaunch(b->exec, ctx->cur));
+    return SIMPLER_OK;
+}
+
+int simpler_batch_destroy(simpler_batch *b) {
+    if (!b) return SIMPLER_ERR_INVALID;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->cur);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    (void)hipFree(b->d_cfgs);
+    (void)hipFree(b->d_prefix);
+    (void)hipFree(b->d_jtab);
+    delete b;
+    return SIMPLER_OK;
+}
+
+}  // extern "C"
+

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 from . import build as _build
-from ._abi import SceneDesc, Stats
+from ._abi import BatchCfg, SceneDesc, Stats
 
 _LIB = None
 
@@ -24,7 +24,9 @@ def load_library():
     global _LIB
     if _LIB is not None:
         return _LIB
-    path = _build.hip_library_path()
+    # SIMPLER_HIP_LIBRARY selects another build of the same ABI (e.g. a
+    # different tuning of the kernels); default: the in-tree library
+    path = os.environ.get("SIMPLER_HIP_LIBRARY") or _build.hip_library_path()
     if not os.path.exists(path):
         raise SimplerError(f"{path} is missing: build it with `python -m simpler_amd.build`")
     L = C.CDLL(path)
@@ -56,11 +58,17 @@ def load_library():
     L.simpler_trace_rays.argtypes = [P, C.c_int64, C.c_uint64, C.c_uint64, P, P, P, P]
     L.simpler_field_eval.argtypes = [P, C.c_int, P, C.c_float, C.c_float, P, P]
     L.simpler_read_stats.argtypes = [P, C.POINTER(Stats)]
+    L.simpler_batch_create.argtypes = [P, C.c_int, C.POINTER(BatchCfg), C.c_int, C.c_int, C.c_int, PP]
+    L.simpler_batch_render.argtypes = [P, P, C.c_int, C.POINTER(Stats)]
+    L.simpler_batch_graph_capture.argtypes = [P, P]
+    L.simpler_batch_graph_launch.argtypes = [P]
+    L.simpler_batch_destroy.argtypes = [P]
     for name in ("simpler_ctx_create", "simpler_ctx_destroy", "simpler_ctx_set_stream", "simpler_ctx_synchronize",
                  "simpler_ctx_set_tuning", "simpler_ctx_get_tuning", "simpler_scene_create", "simpler_scene_destroy",
                  "simpler_scene_update", "simpler_scene_get_desc", "simpler_scene_set_sources",
                  "simpler_scene_set_spline", "simpler_render", "simpler_render_host", "simpler_trace_rays",
-                 "simpler_field_eval", "simpler_read_stats"):
+                 "simpler_field_eval", "simpler_read_stats", "simpler_batch_create", "simpler_batch_render",
+                 "simpler_batch_graph_capture", "simpler_batch_graph_launch", "simpler_batch_destroy"):
         getattr(L, name).restype = C.c_int
     _LIB = L
     return L
@@ -213,6 +221,57 @@ class Scene:
     def close(self):
         if getattr(self, "h", None):
             self.L.simpler_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many transducer configurations (list of dicts with the BatchCfg
+    fields) over one base Bessel scene, rendered by one persistent launch
+    into a (n_cfg, Z, Y, X) float32 CUDA tensor."""
+
+    def __init__(self, scene, configs, res):
+        import torch
+        self.scene = scene
+        self.ctx = scene.ctx
+        self.L = scene.L
+        self.res = res
+        arr = (BatchCfg * len(configs))()
+        for i, c in enumerate(configs):
+            for k, v in c.items():
+                setattr(arr[i], k, v)
+        self.n_cfg = len(configs)
+        self.photons = sum(int(c["n_photons"]) for c in configs)
+        h = C.c_void_p()
+        X, Y, Z = res
+        self.ctx.check(self.L.simpler_batch_create(scene.h, len(configs), arr, X, Y, Z, C.byref(h)))
+        self.h = h
+        self.images = torch.zeros((len(configs), Z, Y, X), dtype=torch.float32, device=f"cuda:{self.ctx.device}")
+        self.captured = False
+
+    def render(self, accumulate=False, stats=True):
+        st = Stats()
+        self.ctx.check(self.L.simpler_batch_render(self.h, C.c_void_p(self.images.data_ptr()), int(accumulate),
+                                                   C.byref(st) if stats else None))
+        return self.images, (st.as_dict() if stats else None)
+
+    def capture(self):
+        """Capture zero + render of self.images into a HIP graph."""
+        self.ctx.check(self.L.simpler_batch_graph_capture(self.h, C.c_void_p(self.images.data_ptr())))
+        self.captured = True
+
+    def replay(self):
+        self.ctx.check(self.L.simpler_batch_graph_launch(self.h))
+        return self.images
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.simpler_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

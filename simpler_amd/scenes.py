@@ -177,3 +177,40 @@ def spline_cube(n=64, sigma_s=2000.0, g=0.9, step=5e-6, res=(64, 64, 32), seed=0
     d.set(field_kind=FIELD_SPLINE, sigma_t=sigma_s, g=g)
     vals, dims, lo, hi = gaussian_spline_coeffs(n=n, seed=seed)
     return d, disk_texture(501), res, (vals, dims, lo, hi)
+
+
+def splitmix64(x):
+    """SplitMix64 finaliser (Steele, Lea, Flood 2014): seed derivation."""
+    x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return x ^ (x >> 31)
+
+
+def batched_configs(n_cfg=256, seed=0, budget_lo=1e5, budget_hi=1e7, scale_budgets=1.0):
+    """Transducer sweep: Delta n ~ U[0, 2e-5], f ~ U[0.5, 2] MHz, mode in
+    {0, 1, 2}, duty-window start phi ~ U[0, 2 pi) (window length 0.4 pi),
+    sigma_s ~ U[0.5, 3] /mm, g ~ U[0.8, 0.95], photon budgets log-uniform in
+    [budget_lo, budget_hi] (times scale_budgets), seeds derived from the batch
+    seed.  Returns (base desc, texture, per-configuration dicts, resolution
+    64 x 64 x 32)."""
+    rng = np.random.default_rng(seed)
+    d = base_desc(step=5e-6)
+    d.set(n_max=1e-5, n_clip=1e-5)
+    cfgs = []
+    for i in range(n_cfg):
+        dn = float(rng.uniform(0.0, 2e-5))
+        phi = float(rng.uniform(0.0, 2 * math.pi))
+        n = int(round(10 ** rng.uniform(math.log10(budget_lo), math.log10(budget_hi)) * scale_budgets))
+        cfgs.append(dict(n_max=max(dn, 1e-12), n_clip=max(dn, 1e-12), f_u=float(rng.uniform(0.5e6, 2e6)),
+                         phi_min=phi, phi_max=phi + 0.4 * math.pi, mode=int(rng.integers(0, 3)),
+                         sigma_t=float(rng.uniform(500.0, 3000.0)), albedo=1.0, g=float(rng.uniform(0.8, 0.95)),
+                         n_photons=max(n, 1), seed=splitmix64(seed * 1000003 + i)))
+    return d, disk_texture(501), cfgs, (64, 64, 32)
+
+
+def desc_for_config(base, cfg):
+    """The single-scene descriptor equivalent to one batched configuration."""
+    d = base.copy()
+    d.set(**{k: v for k, v in cfg.items() if k not in ("n_photons", "seed")})
+    return d
