@@ -155,6 +155,9 @@ void simpler_scene_desc_default(simpler_scene_desc *desc);
  *      thread-safe, separate ctx objects are independent.  Replaces the
  *      process-global __constant__ state of cuda_renderer.cu:28-48. ------ */
 int simpler_ctx_create(int device, simpler_ctx **out);
+/* Fails with SIMPLER_ERR_STATE while scenes or batches of the context are
+ * alive (destroy them first); scene_destroy likewise fails while batches
+ * built on the scene are alive. */
 int simpler_ctx_destroy(simpler_ctx *ctx);
 /* Use an external HIP stream (hipStream_t passed as void*); NULL restores the
  * context's own stream. */

@@ -265,4 +265,249 @@ __device__ __forceinline__ void field_eval<kSpline>(const DevScene &S, const Fie
     g = mk3(vx * S.sxres[0], vy * S.sxres[1], vz * S.sxres[2]);
 }
 
+// ====================================================================
+// Second derivatives and parameter derivatives for the adjoint.
+//
+// field_eval_h: n, grad n and the Hessian of n (the adjoint of a leapfrog
+// step needs H v products).  field_vjp: for scalars a and vector b at a
+// point, adds  a dn/dtheta + b . d(grad n)/dtheta  to the gradient of every
+// field parameter theta the point depends on.
+//   Spline:  theta = the coefficients; into a device grid with the layout of
+//            the coefficients (64 taps, float atomics).
+//   Sources: theta = (a_0..a_{N-1}, phi_0..phi_{N-1}, k); into per-lane
+//            accumulators acc[j * stride] (lane-private LDS).
+// The gap rule makes all of them zero outside [sgap, egap].
+
+struct Sym3 {
+    float xx, yy, zz, xy, xz, yz;
+};
+__device__ __forceinline__ f3 symmul(const Sym3 &H, f3 v) {
+    return mk3(H.xx * v.x + H.xy * v.y + H.xz * v.z, H.xy * v.x + H.yy * v.y + H.yz * v.z,
+               H.xz * v.x + H.yz * v.y + H.zz * v.z);
+}
+__device__ __forceinline__ Sym3 sym_zero() { return Sym3{0.f, 0.f, 0.f, 0.f, 0.f, 0.f}; }
+
+struct GradSink {
+    float *grid;  // spline coefficient gradient (device)
+    float *acc;   // source-parameter accumulators (LDS, lane-private)
+    int stride;
+};
+
+template <int FIELD>
+__device__ __forceinline__ void field_eval_h(const DevScene &S, const FieldP &F, f3 p, float scaling, float tau,
+                                             float &n, f3 &g, Sym3 &H);
+template <int FIELD>
+__device__ __forceinline__ void field_vjp(const DevScene &S, const FieldP &F, f3 p, float scaling, float tau,
+                                          float a, f3 b, const GradSink &G);
+
+// Bessel fields carry no differentiable parameters here; the adjoint entry
+// points reject them.  Defined so that the templates instantiate.
+template <>
+__device__ __forceinline__ void field_eval_h<kBessel>(const DevScene &S, const FieldP &F, f3 p, float scaling,
+                                                      float tau, float &n, f3 &g, Sym3 &H) {
+    field_eval<kBessel>(S, F, p, scaling, tau, n, g);
+    H = sym_zero();
+}
+template <>
+__device__ __forceinline__ void field_vjp<kBessel>(const DevScene &, const FieldP &, f3, float, float, float, f3,
+                                                   const GradSink &) {}
+
+// Point sources.  Per source, with r = |p - x_i|, psi_c = c k r + phi_i - t:
+//   f  = sum_c w_c cos(psi_c) / r
+//   f' = -sum_c w_c (cos + c k r sin) / r^2 = -gs / r^2
+//   f''= (2 gs - hs) / r^3,  hs = sum_c w_c (c k r)^2 cos
+//   grad = a f' rhat,   H = a / r^3 [(3 gs - hs) rhat rhat^T - gs I]
+template <>
+__device__ __forceinline__ void field_eval_h<kSources>(const DevScene &S, const FieldP &F, f3 p, float, float tau,
+                                                       float &n, f3 &g, Sym3 &H) {
+    H = sym_zero();
+    if (p.x > S.egap || p.x < S.sgap) {
+        n = S.n_o;
+        g = mk3(0.f, 0.f, 0.f);
+        return;
+    }
+    float acc = 0.f;
+    f3 gacc = mk3(0.f, 0.f, 0.f);
+    const float tshift = S.time_coupling * tau;
+    for (int i = 0; i < S.n_src; ++i) {
+        const float4 s = S.src[i];
+        const f3 dv = mk3(p.x - s.x, p.y - s.y, p.z - s.z);
+        const float r2 = dot(dv, dv);
+        const float ir = rsqrtf(r2);
+        const float r = r2 * ir;
+        const float ph0 = S.src_phase[i] - tshift;
+        float vs = 0.f, gs = 0.f, hs = 0.f;
+        for (int c = 1; c <= S.n_coeff; ++c) {
+            const float krc = F.k_r * r * (float)c;
+            float sn, cs;
+            __sincosf(krc + ph0, &sn, &cs);
+            const float w = S.coeff_w[c - 1];
+            vs += w * cs;
+            gs += w * (cs + krc * sn);
+            hs += w * krc * krc * cs;
+        }
+        acc += s.w * vs * ir;
+        const float ir3 = ir * ir * ir;
+        const float common = -s.w * gs * ir3;
+        gacc = gacc + common * dv;
+        const float crr = s.w * (3.f * gs - hs) * ir3 * ir * ir;  // times dv dv^T
+        const float cid = -s.w * gs * ir3;                        // times I
+        H.xx += crr * dv.x * dv.x + cid;
+        H.yy += crr * dv.y * dv.y + cid;
+        H.zz += crr * dv.z * dv.z + cid;
+        H.xy += crr * dv.x * dv.y;
+        H.xz += crr * dv.x * dv.z;
+        H.yz += crr * dv.y * dv.z;
+    }
+    n = S.n_o + acc;
+    g = gacc;
+}
+
+// d/da_i:   n: f,            grad: f' rhat
+// d/dphi_i: n: -a ss / r,    grad: a (ss - kc) / r^2 rhat,  ss = sum w sin,
+//                                                           kc = sum w c k r cos
+// d/dk:     n: -a sum w c sin, grad: -a k sum w c^2 cos rhat
+template <>
+__device__ __forceinline__ void field_vjp<kSources>(const DevScene &S, const FieldP &F, f3 p, float, float tau,
+                                                    float a, f3 b, const GradSink &G) {
+    if (p.x > S.egap || p.x < S.sgap) return;
+    const float tshift = S.time_coupling * tau;
+    const int N = S.n_src;
+    float gk = 0.f;
+    for (int i = 0; i < N; ++i) {
+        const float4 s = S.src[i];
+        const f3 dv = mk3(p.x - s.x, p.y - s.y, p.z - s.z);
+        const float r2 = dot(dv, dv);
+        const float ir = rsqrtf(r2);
+        const float r = r2 * ir;
+        const float ph0 = S.src_phase[i] - tshift;
+        float vs = 0.f, gs = 0.f, ss = 0.f, kc = 0.f, cs1 = 0.f, c2c = 0.f;
+        for (int c = 1; c <= S.n_coeff; ++c) {
+            const float fc = (float)c;
+            const float krc = F.k_r * r * fc;
+            float sn, cs;
+            __sincosf(krc + ph0, &sn, &cs);
+            const float w = S.coeff_w[c - 1];
+            vs += w * cs;
+            gs += w * (cs + krc * sn);
+            ss += w * sn;
+            kc += w * krc * cs;
+            cs1 += w * fc * sn;
+            c2c += w * fc * fc * cs;
+        }
+        const float bdr = dot(b, dv) * ir;  // b . rhat
+        // amplitude
+        G.acc[i * G.stride] += a * vs * ir - bdr * gs * ir * ir;
+        // phase
+        G.acc[(N + i) * G.stride] += s.w * (-a * ss * ir + bdr * (ss - kc) * ir * ir);
+        // wavenumber
+        gk += s.w * (-a * cs1 - bdr * F.k_r * c2c);
+    }
+    G.acc[2 * N * G.stride] += gk;
+}
+
+__device__ __forceinline__ void bspline_w2(float t, float w[4], float dw[4], float d2w[4]) {
+    bspline_w(t, w, dw);
+    const float u = 1.0f - t;
+    d2w[0] = u;
+    d2w[1] = 3.0f * t - 2.0f;
+    d2w[2] = 3.0f * u - 2.0f;
+    d2w[3] = t;
+}
+
+template <>
+__device__ __forceinline__ void field_eval_h<kSpline>(const DevScene &S, const FieldP &, f3 p, float, float,
+                                                      float &n, f3 &g, Sym3 &H) {
+    if (p.x > S.egap || p.x < S.sgap) {
+        n = S.n_o;
+        g = mk3(0.f, 0.f, 0.f);
+        H = sym_zero();
+        return;
+    }
+    const float gx = (p.x - S.sxmin[0]) * S.sxres[0];
+    const float gy = (p.y - S.sxmin[1]) * S.sxres[1];
+    const float gz = (p.z - S.sxmin[2]) * S.sxres[2];
+    const float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
+    float wx[4], dwx[4], ddx[4], wy[4], dwy[4], ddy[4], wz[4], dwz[4], ddz[4];
+    bspline_w2(gx - fx, wx, dwx, ddx);
+    bspline_w2(gy - fy, wy, dwy, ddy);
+    bspline_w2(gz - fz, wz, dwz, ddz);
+    const int ix = (int)fx - 1, iy = (int)fy - 1, iz = (int)fz - 1;
+    const int NX = S.sN[0], NY = S.sN[1], NZ = S.sN[2];
+    int xi[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) xi[q] = mirror_idx(ix + q, NX);
+    // s_ab: derivative orders a in x, b in y after the x and y passes
+    float v = 0, vx = 0, vy = 0, vz = 0, vxx = 0, vyy = 0, vzz = 0, vxy = 0, vxz = 0, vyz = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int zi = mirror_idx(iz + c, NZ);
+        float s00 = 0, s10 = 0, s01 = 0, s20 = 0, s02 = 0, s11 = 0;
+#pragma unroll
+        for (int bq = 0; bq < 4; ++bq) {
+            const int yi = mirror_idx(iy + bq, NY);
+            const float *row = S.spline + ((long)zi * NY + yi) * NX;
+            const float c0 = row[xi[0]], c1 = row[xi[1]], c2 = row[xi[2]], c3 = row[xi[3]];
+            const float a0 = c0 * wx[0] + c1 * wx[1] + c2 * wx[2] + c3 * wx[3];
+            const float a1 = c0 * dwx[0] + c1 * dwx[1] + c2 * dwx[2] + c3 * dwx[3];
+            const float a2 = c0 * ddx[0] + c1 * ddx[1] + c2 * ddx[2] + c3 * ddx[3];
+            s00 += a0 * wy[bq];
+            s10 += a1 * wy[bq];
+            s01 += a0 * dwy[bq];
+            s20 += a2 * wy[bq];
+            s02 += a0 * ddy[bq];
+            s11 += a1 * dwy[bq];
+        }
+        v += s00 * wz[c];
+        vx += s10 * wz[c];
+        vy += s01 * wz[c];
+        vz += s00 * dwz[c];
+        vxx += s20 * wz[c];
+        vyy += s02 * wz[c];
+        vzz += s00 * ddz[c];
+        vxy += s11 * wz[c];
+        vxz += s10 * dwz[c];
+        vyz += s01 * dwz[c];
+    }
+    const float rx = S.sxres[0], ry = S.sxres[1], rz = S.sxres[2];
+    n = S.spline_ref + v;
+    g = mk3(vx * rx, vy * ry, vz * rz);
+    H = Sym3{vxx * rx * rx, vyy * ry * ry, vzz * rz * rz, vxy * rx * ry, vxz * rx * rz, vyz * ry * rz};
+}
+
+// coefficient (x, y, z) receives a wx wy wz + bx rx dwx wy wz + by ry wx dwy wz
+// + bz rz wx wy dwz  (mirror-wrapped taps may coincide: atomics add them).
+template <>
+__device__ __forceinline__ void field_vjp<kSpline>(const DevScene &S, const FieldP &, f3 p, float, float, float a,
+                                                   f3 b, const GradSink &G) {
+    if (p.x > S.egap || p.x < S.sgap) return;
+    const float gx = (p.x - S.sxmin[0]) * S.sxres[0];
+    const float gy = (p.y - S.sxmin[1]) * S.sxres[1];
+    const float gz = (p.z - S.sxmin[2]) * S.sxres[2];
+    const float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
+    float wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
+    bspline_w(gx - fx, wx, dwx);
+    bspline_w(gy - fy, wy, dwy);
+    bspline_w(gz - fz, wz, dwz);
+    const int ix = (int)fx - 1, iy = (int)fy - 1, iz = (int)fz - 1;
+    const int NX = S.sN[0], NY = S.sN[1], NZ = S.sN[2];
+    const float bx = b.x * S.sxres[0], by = b.y * S.sxres[1], bz = b.z * S.sxres[2];
+    int xi[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) xi[q] = mirror_idx(ix + q, NX);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int zi = mirror_idx(iz + c, NZ);
+#pragma unroll
+        for (int bq = 0; bq < 4; ++bq) {
+            const int yi = mirror_idx(iy + bq, NY);
+            float *row = G.grid + ((long)zi * NY + yi) * NX;
+            const float Pw = (a * wy[bq] + by * dwy[bq]) * wz[c] + bz * wy[bq] * dwz[c];  // times wx
+            const float Pd = bx * wy[bq] * wz[c];                                          // times dwx
+#pragma unroll
+            for (int q = 0; q < 4; ++q) unsafeAtomicAdd(row + xi[q], Pw * wx[q] + Pd * dwx[q]);
+        }
+    }
+}
+
 }  // namespace simpler

@@ -746,6 +746,7 @@ struct simpler_ctx {
     unsigned long long *d_scratch = nullptr;  // [0..7] work, [8..15] counters
     int num_cu = 256;
     int block = 256, spc = 32, bpc = 4;
+    int live = 0;  // scenes and batches not yet destroyed
     std::string err;
 };
 
@@ -766,6 +767,7 @@ struct simpler_scene {
     int sN[3] = {0, 0, 0};
     float sxmin[3] = {0, 0, 0}, sxmax[3] = {0, 0, 0};
     float spline_min = 0.f;
+    int batches = 0;  // batches built on this scene
 };
 
 static int fail(simpler_ctx *ctx, int code, const std::string &msg) {
@@ -1109,6 +1111,7 @@ int simpler_ctx_create(int device, simpler_ctx **out) {
 
 int simpler_ctx_destroy(simpler_ctx *ctx) {
     if (!ctx) return SIMPLER_ERR_INVALID;
+    if (ctx->live > 0) return fail(ctx, SIMPLER_ERR_STATE, "destroy the scenes and batches of this context first");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->cur);
     (void)hipFree(ctx->d_scratch);
@@ -1204,12 +1207,15 @@ int simpler_scene_create(simpler_ctx *ctx, const simpler_scene_desc *desc, const
             return rc2;
         }
     }
+    ++ctx->live;
     *out = sc;
     return SIMPLER_OK;
 }
 
 int simpler_scene_destroy(simpler_scene *sc) {
     if (!sc) return SIMPLER_ERR_INVALID;
+    if (sc->batches > 0) return fail(sc->ctx, SIMPLER_ERR_STATE, "destroy the batches built on this scene first");
+    --sc->ctx->live;
     (void)hipSetDevice(sc->ctx->device);
     (void)hipStreamSynchronize(sc->ctx->cur);
     (void)hipFree(sc->d_cdf);
@@ -1459,6 +1465,7 @@ struct simpler_batch {
     DevScene S;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    bool counted = false;  // registered with ctx->live / base->batches
 };
 
 static int batch_launch(simpler_batch *b, float *images, int accumulate, hipStream_t stream) {
@@ -1572,6 +1579,9 @@ int simpler_batch_create(simpler_scene *base, int n_cfg, const simpler_batch_cfg
         return fail(ctx, SIMPLER_ERR_HIP, "batch: upload failed");
     }
     b->S = make_dev_scene(base, 1, 0, false);
+    ++ctx->live;
+    ++base->batches;
+    b->counted = true;
     *out = b;
     return SIMPLER_OK;
 }
@@ -1623,6 +1633,10 @@ int simpler_batch_destroy(simpler_batch *b) {
     (void)hipStreamSynchronize(b->ctx->cur);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->counted) {
+        --b->ctx->live;
+        --b->base->batches;
+    }
     (void)hipFree(b->d_cfgs);
     (void)hipFree(b->d_prefix);
     (void)hipFree(b->d_jtab);

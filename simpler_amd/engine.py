@@ -6,6 +6,7 @@ constructing a Context without the built library or without a GPU raises.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -88,6 +89,8 @@ class Context:
             raise SimplerError("no GPU visible: the engine has no CPU fallback")
         self.L = load_library()
         self.device = device
+        self._scenes = weakref.WeakSet()
+        self._batches = weakref.WeakSet()
         torch.cuda.set_device(device)
         h = C.c_void_p()
         rc = self.L.simpler_ctx_create(device, C.byref(h))
@@ -122,8 +125,13 @@ class Context:
         return st.as_dict()
 
     def close(self):
+        """Destroys the context after its batches and scenes."""
         if getattr(self, "h", None):
-            self.L.simpler_ctx_destroy(self.h)
+            for b in list(self._batches):
+                b.close()
+            for sc in list(self._scenes):
+                sc.close()
+            self.check(self.L.simpler_ctx_destroy(self.h))
             self.h = None
 
     def __del__(self):
@@ -150,6 +158,8 @@ class Scene:
                                              C.byref(h))
         ctx.check(rc)
         self.h = h
+        self._batches = weakref.WeakSet()
+        ctx._scenes.add(self)
         if sources is not None:
             self.set_sources(**sources)
         if spline is not None:
@@ -219,8 +229,11 @@ class Scene:
         return n, g
 
     def close(self):
+        """Destroys the scene after the batches built on it."""
         if getattr(self, "h", None):
-            self.L.simpler_scene_destroy(self.h)
+            for b in list(self._batches):
+                b.close()
+            self.ctx.check(self.L.simpler_scene_destroy(self.h))
             self.h = None
 
     def __del__(self):
@@ -251,6 +264,8 @@ class Batch:
         X, Y, Z = res
         self.ctx.check(self.L.simpler_batch_create(scene.h, len(configs), arr, X, Y, Z, C.byref(h)))
         self.h = h
+        scene._batches.add(self)
+        self.ctx._batches.add(self)
         self.images = torch.zeros((len(configs), Z, Y, X), dtype=torch.float32, device=f"cuda:{self.ctx.device}")
         self.captured = False
 

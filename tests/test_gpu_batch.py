@@ -62,3 +62,14 @@ def test_batch_configuration_matches_oracle(gpu_ctx):
         ref, _ = O.OracleScene(scenes.desc_for_config(d, c), tex).render(
             64, res, seed=c["seed"], precision=32, threads=gpu_threads(), cutoff=True)
         assert abs(imgs[i].sum() - ref.sum()) <= 2e-3 * ref.sum(), i
+
+
+def test_destroy_order_is_enforced():
+    ctx = engine.Context(0)
+    d, tex, cfgs, res = small_batch(n_cfg=2)
+    sc = engine.Scene(ctx, d, tex)
+    b = engine.Batch(sc, cfgs, res)
+    assert ctx.L.simpler_ctx_destroy(ctx.h) == 3  # SIMPLER_ERR_STATE: scene and batch alive
+    assert ctx.L.simpler_scene_destroy(sc.h) == 3  # batch alive
+    ctx.close()  # closes the batch, then the scene, then the context
+    assert b.h is None and sc.h is None and ctx.h is None
