@@ -146,6 +146,74 @@ def batch_bench(args):
     return out
 
 
+def adjoint_bench(args):
+    """Differentiable spline field: 64^3 coefficients on [-5, 5]^3 mm (n0 +
+    seeded noise, Gaussian-filtered sigma = 4 cells, max |dn| 5e-5), sigma_s
+    2/mm, g 0.9, 5 um ER step, 64x64x32 sensor.  Objective: throughput into a
+    1 mm aperture.  Times the forward render and the path-replay adjoint
+    (gradient w.r.t. all 262,144 coefficients) on the same photons, and checks
+    the adjoint against central differences with common random numbers on K
+    coefficients drawn at random among the top 10 % by gradient magnitude."""
+    import numpy as np
+    import torch
+
+    from simpler_amd import engine, scenes
+
+    ctx = engine.Context(0)
+    d, tex, res, (coeff, dims, lo, hi) = scenes.spline_cube_ready(n=64)
+    sc = engine.Scene(ctx, d, tex, spline=(coeff, dims, lo, hi))
+    G = torch.tensor(scenes.aperture_weights(res), device="cuda")
+    N = args.adjoint_photons
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        out = fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return out, e0.elapsed_time(e1) / 1e3
+
+    sc.render(4096, res, seed=11)
+    sc.adjoint_render(G, 4096, seed=11)
+    (img, fst), t_fwd = timed(lambda: sc.render(N, res, seed=11))
+    F_img = float((img.double() * G.double()).sum())
+    (grad, F, ast), t_adj = timed(lambda: sc.adjoint_render(G, N, seed=11))
+    grad = grad.cpu().numpy().astype(np.float64)
+    eprint(f"adjoint: {N} paths, forward {t_fwd:.2f}s, adjoint {t_adj:.2f}s, F {F:.6g} (render {F_img:.6g})")
+    rng = np.random.default_rng(3)
+    top = np.argsort(-np.abs(grad))[: max(args.adjoint_fd, len(grad) // 10)]
+    picks = rng.choice(top, args.adjoint_fd, replace=False)
+    errs, t_fd = [], 0.0
+    for j in picks:
+        Fs = []
+        for sgn in (1, -1):
+            c = coeff.copy()
+            c[j] = np.float32(coeff[j] + sgn * args.adjoint_eps)
+            sc.set_spline(c, dims, lo, hi)
+            (im, _), t = timed(lambda: sc.render(N, res, seed=11, stats=False))
+            t_fd += t
+            Fs.append((float(c[j]), float((im.double() * G.double()).sum())))
+        fd = (Fs[0][1] - Fs[1][1]) / (Fs[0][0] - Fs[1][0])
+        errs.append(abs(grad[j] - fd) / abs(fd) if fd != 0 else float("inf"))
+        eprint(f"  coefficient {j}: adjoint {grad[j]:.5g}  central difference {fd:.5g}  rel err {errs[-1]:.3g}")
+    sc.set_spline(coeff, dims, lo, hi)
+    n_par = len(grad)
+    per_fd = t_fd / max(1, len(picks))
+    out = dict(workload=("64^3 spline field, sigma_s 2/mm, g 0.9, 5 um step, 64x64x32 sensor, "
+                         "throughput into a 1 mm aperture"),
+               paths=N, parameters=n_par, forward_s=t_fwd, adjoint_s=t_adj, adjoint_over_forward=t_adj / t_fwd,
+               objective=F, objective_render=F_img, forward_paths_per_s=N / t_fwd,
+               fd_check=dict(coefficients=[int(j) for j in picks], rel_err=errs,
+                             median_rel_err=float(np.median(errs)), max_rel_err=float(np.max(errs)),
+                             eps=args.adjoint_eps, seconds_per_coefficient=per_fd),
+               fd_full_gradient_s=per_fd * n_par, adjoint_speedup_over_fd=per_fd * n_par / t_adj,
+               adjoint_stats=ast, forward_er_steps=fst["er_steps"])
+    sc.close()
+    ctx.close()
+    return out
+
+
 def cpu_baseline(args):
     from oracle import oracle as orc
     from simpler_amd import scenes
@@ -190,6 +258,10 @@ def main():
     ap.add_argument("--batch-scale", type=float, default=0.02,
                     help="photon budgets of the batched sweep times this factor (1.0: 5.5e8 photons)")
     ap.add_argument("--batch-replays", type=int, default=2)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-photons", type=int, default=1 << 20)
+    ap.add_argument("--adjoint-fd", type=int, default=8, help="coefficients checked by central differences")
+    ap.add_argument("--adjoint-eps", type=float, default=5e-5)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -231,6 +303,8 @@ def main():
     }
     if not args.no_batch:
         out["batched"] = batch_bench(args)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

@@ -287,6 +287,52 @@ int simpler_batch_graph_capture(simpler_batch *batch, float *images_dev);
 int simpler_batch_graph_launch(simpler_batch *batch);
 int simpler_batch_destroy(simpler_batch *batch);
 
+/* ---- path-replay adjoint (no counterpart in the reference, which has no
+ *      gradient with respect to field parameters; its forward-mode
+ *      derivative machinery is scene.cpp:351-359 / scene.h:259-272).
+ *
+ * Differentiable parameters theta of the scene's field:
+ *   spline         the N0*N1*N2 coefficients, layout of simpler_scene_set_spline;
+ *   point sources  2*n_src + 1 values: amplitudes a_i, phases phi_i, then the
+ *                  wavenumber k = 2 pi f_u / speed_u (n_src <= 16);
+ *   Bessel         none (the adjoint calls fail).
+ * Every photon is re-traced from its seed exactly as by simpler_render; paths
+ * are differentiated with their random numbers held fixed (the Fresnel
+ * transmission factor and the discrete choices -- time bin, face, reflect or
+ * refract -- are held fixed too).  Scenes with an active sensor lens or
+ * precision > 7 are rejected.  grad_dev (float, n_params) is overwritten, or
+ * added to with accumulate = 1.  objective_host / stats_host may be NULL;
+ * when given, the call synchronizes and fills them. ------------------------ */
+typedef struct simpler_adjoint_stats {
+    uint64_t photons;
+    uint64_t er_steps;             /* forward ER steps                          */
+    uint64_t reverse_steps;        /* backward (adjoint) ER steps               */
+    uint64_t sub_paths_reversed;   /* NEE / direct / per-ray sub-paths          */
+    uint64_t main_paths_reversed;
+    uint64_t dropped;              /* contributions skipped: segment log full   */
+} simpler_adjoint_stats;
+
+int simpler_adjoint_param_count(const simpler_scene *scene, int64_t *n_params);
+
+/* Gradient of F = sum_pixels dF_dimage[pixel] * image[pixel], image being the
+ * render of simpler_render with the same arguments (dF_dimage: device float
+ * array, same layout and resolution); objective_host receives F (double). */
+int simpler_adjoint_render(simpler_scene *scene, int64_t n_photons, int64_t n_total,
+                           uint64_t seed, uint64_t photon_offset,
+                           const float *dF_dimage_dev, int x_res, int y_res, int z_res,
+                           float *grad_dev, int accumulate, double *objective_host,
+                           simpler_adjoint_stats *stats_host);
+
+/* Gradient of F = sum_rays g_pos . exit_pos + g_dir . exit_dir + g_len * len
+ * over the rays of simpler_trace_rays with the same arguments (device arrays
+ * 3, 3 and 1 floats per ray); rays that do not reach the sensor face add
+ * nothing. */
+int simpler_adjoint_rays(simpler_scene *scene, int64_t n_rays, uint64_t seed,
+                         uint64_t photon_offset, const float *g_pos_dev,
+                         const float *g_dir_dev, const float *g_len_dev,
+                         float *grad_dev, int accumulate, double *objective_host,
+                         simpler_adjoint_stats *stats_host);
+
 /* Tunables of the persistent march kernel (picked by the autotuner):
  * block_size in {64,128,256,512}, steps_per_chunk >= 1 (march steps between
  * event phases), blocks_per_cu >= 1.  0 keeps the current value. */

@@ -116,6 +116,22 @@ class Stats(C.Structure):
         return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
+class AdjointStats(C.Structure):
+    """``simpler_adjoint_stats``: work counters of one adjoint call."""
+
+    _fields_ = [
+        ("photons", C.c_uint64),
+        ("er_steps", C.c_uint64),
+        ("reverse_steps", C.c_uint64),
+        ("sub_paths_reversed", C.c_uint64),
+        ("main_paths_reversed", C.c_uint64),
+        ("dropped", C.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+
+
 class BatchCfg(C.Structure):
     """``simpler_batch_cfg``: per-configuration overrides of a batched launch."""
 

@@ -1,0 +1,194 @@
+// adjoint_kernel.h -- the backward-run events and the persistent adjoint
+// kernel (included by render.hip after the path events it resumes).
+#pragma once
+
+namespace simpler {
+
+// Load a logged segment end as the current reverse point.
+template <int FIELD>
+__device__ __forceinline__ void rev_load(Lane &L, const float *base, int idx, f3 &lamp, f3 &lamv, uint32_t &cnt0,
+                                         uint32_t &pat, uint32_t &kind) {
+    AdjState &a = *L.a;
+    Hot &h = L.h;
+    f3 p, v;
+    rec_read(base, idx, p, v, lamp, lamv, cnt0, pat, kind);
+    h.p = p;
+    h.v = v;
+    field_eval_h<FIELD>(L.S, L.v.f, p, h.scaling, h.tau, h.n0, h.g0, a.H);
+    a.pa = 0.f;
+    a.pb = mk3(0.f, 0.f, 0.f);
+}
+
+// Photon end with adjoints on its vertices: start the backward run of the main
+// path at the last such vertex.  mu is minus the adjoint of the flight's
+// distance budget, which moves the vertex along the path.
+template <int FIELD>
+__device__ void main_rev_init(Lane &L) {
+    AdjState &a = *L.a;
+    Hot &h = L.h;
+    a.rphase = RP_MAIN;
+    a.rseg = a.last_lam;
+    a.gL = 0.f;
+    f3 lamp, lamv;
+    uint32_t cnt0, pat, kind;
+    rev_load<FIELD>(L, a.log_main, a.rseg, lamp, lamv, cnt0, pat, kind);
+    a.lp = lamp;
+    a.lv = lamv;
+    a.mu = -(dot(a.lp, h.v / h.n0) + dot(a.lv, h.g0));
+    ++a.nmainrev;
+    rev_init(h, a, L.S.h0, cnt0, pat);
+}
+
+// The backward run reached the start of a segment.
+template <int FIELD, int MODE>
+__device__ void rev_end(Lane &L) {
+    AdjState &a = *L.a;
+    Hot &h = L.h;
+    const DevScene &S = L.S;
+    const FieldP &F = L.v.f;
+    field_vjp<FIELD>(S, F, h.p, h.scaling, h.tau, a.pa, a.pb, a.G);
+    f3 lamp, lamv;
+    uint32_t cnt0, pat, kind;
+    if (a.rphase == RP_MAIN) {
+        if (a.rseg == 0) {  // the source: v0 = n(p0) d0 with p0, d0 fixed
+            const float vl = len(h.v);
+            field_vjp<FIELD>(S, F, h.p, h.scaling, h.tau, dot(a.lv, h.v) / vl, mk3(0.f, 0.f, 0.f), a.G);
+            a.pending = false;
+            h.state = ST_NEED;
+            return;
+        }
+        const f3 vout = h.v;
+        --a.rseg;
+        rev_load<FIELD>(L, a.log_main, a.rseg, lamp, lamv, cnt0, pat, kind);
+        if (kind & K_FACE) {  // mirror reflection, then the face-crossing time
+            const f3 nrm = face_normal((int)(kind >> 1));
+            a.lv = reflect_vec(a.lv, nrm);
+            face_correct(a, nrm, h.v / h.n0, h.g0, a.mu);
+        } else {  // Henyey-Greenstein vertex: a new flight begins here
+            a.lv = hg_vjp(h.v, vout, a.lv);
+            a.lp = a.lp + lamp;
+            a.lv = a.lv + lamv;
+            a.mu = -(dot(a.lp, h.v / h.n0) + dot(a.lv, h.g0));
+        }
+        rev_init(h, a, S.h0, cnt0, pat);
+        return;
+    }
+    if (a.rseg > 0) {  // a face reflection inside the sub-path
+        --a.rseg;
+        rev_load<FIELD>(L, a.log_sub, a.rseg, lamp, lamv, cnt0, pat, kind);
+        const f3 nrm = face_normal((int)(kind >> 1));
+        a.lv = reflect_vec(a.lv, nrm);
+        face_correct(a, nrm, h.v / h.n0, h.g0, a.mu + a.gL * h.n0);
+        rev_init(h, a, S.h0, cnt0, pat);
+        return;
+    }
+    // sub-path start: v0 = n(p0) omega with omega fixed
+    const float vl = len(h.v);
+    const float lw = dot(a.lv, h.v) / vl;
+    field_vjp<FIELD>(S, F, h.p, h.scaling, h.tau, lw, mk3(0.f, 0.f, 0.f), a.G);
+    a.gL = 0.f;
+    if constexpr (MODE == M_ADJ) {
+        if (a.rphase == RP_NEE) {
+            rec_add_lambda(a.log_main, a.nmain - 1, a.lp + lw * h.g0, a.ldin);
+            a.pending = true;
+            a.last_lam = a.nmain - 1;
+            nee_continue<FIELD>(L);
+        } else {
+            direct_continue<FIELD>(L, 1);
+        }
+    } else {
+        photon_end(L);
+    }
+}
+
+// The persistent adjoint kernel: the render kernel's fetch / events / march
+// loop plus the backward events (ST_REND, ST_MREV) and backward steps.
+template <int FIELD, int MODE>
+__global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
+    __shared__ Cold cold[kBlock];
+    __shared__ uint32_t cnt[C_N];
+    __shared__ float sacc[FIELD == kSources ? (2 * kMaxAdjSources + 1) * kBlock : 1];
+    if (threadIdx.x < C_N) cnt[threadIdx.x] = 0u;
+    const int npar = FIELD == kSources ? 2 * S.n_src + 1 : 0;
+    for (int j = 0; j < npar; ++j) sacc[j * kBlock + threadIdx.x] = 0.f;
+    __syncthreads();
+    Var v = var_from(S, A);
+    Hot h;
+    h.steps = 0;
+    h.state = ST_NEED;
+    h.dead = false;
+    AdjState a;
+    a.F = 0.0;
+    a.rsteps = a.nsubrev = a.nmainrev = a.dropped = 0;
+    a.pending = a.trunc_main = a.trunc_sub = false;
+    a.gL = 0.f;
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    a.log_main = A.log + gid * (size_t)(A.main_cap + A.sub_cap) * kRecFloats;
+    a.log_sub = a.log_main + (size_t)A.main_cap * kRecFloats;
+    a.G.grid = A.grad;
+    a.G.acc = sacc + threadIdx.x;
+    a.G.stride = kBlock;
+    Lane L{h, cold[threadIdx.x], v, S, A, cnt, &a};
+    const int lane = __lane_id();
+    while (true) {
+        const unsigned long long need = __ballot(h.state == ST_NEED);
+        if (need) {
+            const int leader = __ffsll((long long)need) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(A.work, (unsigned long long)__popcll(need));
+            base = __shfl(base, leader);
+            if (h.state == ST_NEED) {
+                const unsigned long long rank = __popcll(need & ((1ull << lane) - 1ull));
+                const unsigned long long idx = base + rank;
+                if (idx < A.n) {
+                    const uint64_t pid = A.offset + idx;
+                    L.c.pid_lo = (uint32_t)pid;
+                    L.c.pid_hi = (uint32_t)(pid >> 32);
+                    h.state = ST_INIT;
+                } else {
+                    h.state = ST_DONE;
+                }
+            }
+        }
+        if (h.state == ST_INIT) photon_init<FIELD, MODE>(L);
+        if (h.state == ST_SEG_END) seg_end<FIELD, MODE>(L);
+        if (h.state == ST_REND) rev_end<FIELD, MODE>(L);
+        if (h.state == ST_MREV) main_rev_init<FIELD>(L);
+        if (__ballot(h.state != ST_DONE) == 0ull) break;
+        if (__ballot(h.state == ST_MARCH || h.state == ST_RMARCH) == 0ull) continue;
+        const FieldP F = v.f;
+        const float Lcut = v.Lcut;
+        for (int k = 0; k < A.spc; ++k) {
+            if (h.state == ST_MARCH)
+                march_step<FIELD, true>(h, S, F, Lcut);
+            else if (h.state == ST_RMARCH)
+                rev_step<FIELD>(h, a, S, F);
+        }
+    }
+    unsigned long long st = h.steps;
+    double fo = a.F;
+    unsigned long long ac[A_N] = {a.rsteps, a.nsubrev, a.nmainrev, a.dropped};
+    for (int off = 32; off > 0; off >>= 1) {
+        st += __shfl_xor(st, off);
+        fo += __shfl_xor(fo, off);
+#pragma unroll
+        for (int k = 0; k < A_N; ++k) ac[k] += __shfl_xor(ac[k], off);
+    }
+    if (lane == 0) {
+        if (st) atomicAdd(A.counters + C_STEPS, st);
+        if (fo != 0.0) atomicAdd(A.obj, fo);
+#pragma unroll
+        for (int k = 0; k < A_N; ++k)
+            if (ac[k]) atomicAdd(A.acnt + k, ac[k]);
+    }
+    __syncthreads();
+    if (threadIdx.x < C_N && threadIdx.x != C_STEPS && cnt[threadIdx.x])
+        atomicAdd(A.counters + threadIdx.x, (unsigned long long)cnt[threadIdx.x]);
+    for (int j = threadIdx.x; j < npar; j += kBlock) {
+        float sum = 0.f;
+        for (int t = 0; t < kBlock; ++t) sum += sacc[j * kBlock + t];
+        if (sum != 0.f) atomicAdd(A.grad + j, sum);
+    }
+}
+
+}  // namespace simpler

@@ -36,7 +36,12 @@
 namespace simpler {
 
 enum { PH_DIRECT = 0, PH_NEE = 1, PH_FLIGHT = 2 };
-enum { ST_MARCH = 0, ST_SEG_END = 1, ST_NEED = 2, ST_INIT = 3, ST_DONE = 4 };
+enum { ST_MARCH = 0, ST_SEG_END = 1, ST_NEED = 2, ST_INIT = 3, ST_DONE = 4, ST_RMARCH = 5, ST_REND = 6, ST_MREV = 7 };
+// kernel modes: image render, per-ray record, adjoint of an image objective,
+// adjoint of a per-ray objective
+enum { M_RENDER = 0, M_RECORD = 1, M_ADJ = 2, M_ADJRAYS = 3 };
+__host__ __device__ constexpr bool per_ray(int m) { return m == M_RECORD || m == M_ADJRAYS; }
+__host__ __device__ constexpr bool is_adj(int m) { return m == M_ADJ || m == M_ADJRAYS; }
 enum { C_PHOT = 0, C_STEPS, C_SCAT, C_NEE, C_DIRECT, C_FACE, C_SEG, C_ABORT, C_N };
 constexpr uint32_t kMaxFaceEvents = 1u << 20;  // per march; see the abort counter
 constexpr int kBlock = 256;
@@ -54,6 +59,15 @@ struct RenderArgs {
     const struct Var *cfgs;
     const unsigned long long *prefix;  // n_cfg + 1 entries
     int n_cfg;
+    // adjoint: objective weights, parameter gradient, objective value,
+    // lane-private segment logs, per-ray upstream gradients, counters
+    const float *G;
+    float *grad;
+    double *obj;
+    float *log;
+    int main_cap, sub_cap;
+    const float *up_p, *up_d, *up_L;
+    unsigned long long *acnt;
 };
 
 // Hot photon state: what the march loop reads and writes every step; kept in
@@ -64,6 +78,7 @@ struct Hot {
     float travelled, seg_dist, h, scaling, tau;
     int prec;
     uint32_t steps_left, steps;
+    uint32_t fcnt0, fpat;  // step pattern of the segment (adjoint only)
     int state;
     bool dead;
 };
@@ -110,7 +125,8 @@ struct Lane {
     Var &v;
     const DevScene &S;
     const RenderArgs &A;
-    uint32_t *cnt;  // block-level event counters in LDS
+    uint32_t *cnt;          // block-level event counters in LDS
+    struct AdjState *a;     // adjoint state (adjoint kernels only)
 };
 
 __device__ __forceinline__ void count(Lane &L, int k) { atomicAdd(L.cnt + k, 1u); }
@@ -140,6 +156,8 @@ __device__ __forceinline__ void start_segment(Lane &L, float dist) {
     h.steps_left = q >= 4294967040.0f ? 0xFFFFFFFFu : (uint32_t)q;
     h.dead = false;
     h.state = ST_MARCH;
+    h.fcnt0 = 0;
+    h.fpat = 0;
     count(L, C_SEG);
 }
 
@@ -160,7 +178,9 @@ __device__ __forceinline__ bool inside_block(const DevScene &S, f3 p) {
 // (scene.cpp:323-330): kick (h/2) grad n(p), drift h v / n(p), kick
 // (h/2) grad n(p').  (n, grad n) at the start point is the previous step's
 // end-point evaluation.
-template <int FIELD>
+// LOG: also record the step pattern (accepted steps per refinement level,
+// final level, kept overshoot step) that the adjoint replays backwards.
+template <int FIELD, bool LOG = false>
 __device__ __forceinline__ void march_step(Hot &h, const DevScene &S, const FieldP &F, float Lcut) {
     const float hs = h.h;
     const float hh = 0.5f * hs;
@@ -177,6 +197,7 @@ __device__ __forceinline__ void march_step(Hot &h, const DevScene &S, const Fiel
     field_eval<FIELD>(S, F, p1, h.scaling, h.tau, n1, g1);
     const f3 v1 = vh + hh * g1;
     ++h.steps;
+    const int lev = S.precision - h.prec;
     const bool reject = !inside_block(S, p1) || (h.travelled + hs) > h.seg_dist;
     const bool stop = reject && (--h.prec < 0);  // past the face / distance (scene.cpp:405-407)
     if (!reject || stop) {
@@ -187,6 +208,7 @@ __device__ __forceinline__ void march_step(Hot &h, const DevScene &S, const Fiel
         h.g0 = g1;
     }
     if (stop) {
+        if (LOG) h.fpat |= ((uint32_t)lev << 28) | (1u << 31);
         end_segment(h);
     } else if (reject) {  // restore and refine: h/10, 10 more steps (scene.cpp:408-412)
         h.h = hs / 10.0f;
@@ -194,10 +216,17 @@ __device__ __forceinline__ void march_step(Hot &h, const DevScene &S, const Fiel
     } else {
         h.travelled += hs;
         comp_add(h.L, h.Ll, hs * n1);  // optical length (scene.cpp:416)
+        if (LOG) {
+            if (lev == 0)
+                ++h.fcnt0;
+            else
+                h.fpat += 1u << (4 * (lev - 1));
+        }
         if (h.L > Lcut) {
             h.dead = true;
             end_segment(h);
         } else if (--h.steps_left == 0) {
+            if (LOG) h.fpat |= (uint32_t)lev << 28;
             end_segment(h);
         }
     }
@@ -352,7 +381,14 @@ __device__ __forceinline__ bool exit_to_sensor(const DevScene &S, f3 &p, f3 d, f
     return true;
 }
 
-__device__ __forceinline__ void photon_end(Lane &L) { L.h.state = ST_NEED; }
+struct AdjState;
+__device__ __forceinline__ bool adj_main_pending(const AdjState *a);
+
+// End of a photon.  In the adjoint kernels a photon whose vertices received
+// adjoints first runs its main path backwards (ST_MREV).
+__device__ __forceinline__ void photon_end(Lane &L) {
+    L.h.state = (L.a && adj_main_pending(L.a)) ? ST_MREV : ST_NEED;
+}
 
 __device__ __forceinline__ void flight_start(Lane &L, float dist) {
     L.c.phase = PH_FLIGHT;
@@ -398,6 +434,10 @@ __device__ __forceinline__ void restore_main(Lane &L, float Lval) {
     L.c.draw = L.c.main_draw;
 }
 
+}  // namespace simpler
+#include "adjoint.h"
+namespace simpler {
+
 // photon.cpp:111-116 loop head + scene.cpp:866-893 (NEE start)
 __device__ void vertex(Lane &L) {
     const DevScene &S = L.S;
@@ -424,18 +464,45 @@ __device__ void vertex(Lane &L) {
         c.acc = 0.f;
         c.large = kLargeDist;
         c.faces = 0;
+        if (L.a) adj_sub_begin(*L.a);
         start_segment(L, kLargeDist);
         return;
     }
     scatter_once(L);
 }
 
-template <int FIELD, bool RECORD>
+// After the direct trace (and its adjoint): back to the main path
+// (photon.cpp:331-333).
+template <int FIELD>
+__device__ void direct_continue(Lane &L, int res) {
+    const DevScene &S = L.S;
+    if (res == -2 && S.direct_leak) {  // the leaked length is already dead
+        photon_end(L);
+        return;
+    }
+    restore_main(L, S.direct_leak ? L.h.L : L.c.mL);
+    refresh_field<FIELD>(L);
+    scatter_start(L);
+}
+
+template <int FIELD>
+__device__ void nee_continue(Lane &L) {
+    restore_main(L, L.c.mL);
+    refresh_field<FIELD>(L);
+    scatter_once(L);
+}
+
+template <int FIELD, int MODE>
 __device__ void direct_done(Lane &L, int res) {
     const DevScene &S = L.S;
     Hot &h = L.h;
     Cold &c = L.c;
-    if (RECORD) {
+    if constexpr (MODE == M_ADJRAYS) {
+        if (res == 1 && ray_adj_start<FIELD>(L)) return;
+        photon_end(L);
+        return;
+    }
+    if constexpr (MODE == M_RECORD) {
         const RenderArgs &A = L.A;
         const size_t i = (size_t)((((uint64_t)c.pid_hi << 32) | c.pid_lo) - A.offset);
         const f3 d = normalized(h.v);
@@ -456,19 +523,18 @@ __device__ void direct_done(Lane &L, int res) {
         if (exit_to_sensor(S, p, h.v, h.n0, Lv, fore, fw, dts)) {
             h.L = Lv;
             count(L, C_DIRECT);
-            splat(S, L.A, L.v.image, p, Lv, 0, L.v.weight * expf(-L.v.sigma_t * c.acc) * fw);
+            const float val = L.v.weight * expf(-L.v.sigma_t * c.acc) * fw;
+            if constexpr (MODE == M_ADJ) {
+                if (direct_adj_start<FIELD>(L, p, Lv, val)) return;
+            } else {
+                splat(S, L.A, L.v.image, p, Lv, 0, val);
+            }
         }
     }
-    if (res == -2 && S.direct_leak) {  // the leaked length is already dead
-        photon_end(L);
-        return;
-    }
-    restore_main(L, S.direct_leak ? h.L : c.mL);  // photon.cpp:331-333
-    refresh_field<FIELD>(L);
-    scatter_start(L);
+    direct_continue<FIELD>(L, res);
 }
 
-template <int FIELD>
+template <int FIELD, int MODE>
 __device__ void nee_done(Lane &L, int res) {
     const DevScene &S = L.S;
     Hot &h = L.h;
@@ -481,17 +547,19 @@ __device__ void nee_done(Lane &L, int res) {
             const float val =
                 L.v.weight * (2.f * kPi) * expf(-L.v.sigma_t * c.acc) * hg_f(L.v.g, din, dts) * fore * fw;
             count(L, C_NEE);
-            splat(S, L.A, L.v.image, p, Lv, c.depth, val);
+            if constexpr (MODE == M_ADJ) {
+                if (nee_adj_start<FIELD>(L, p, Lv, val, din, dts, fore)) return;
+            } else {
+                splat(S, L.A, L.v.image, p, Lv, c.depth, val);
+            }
         }
     }
-    restore_main(L, c.mL);
-    refresh_field<FIELD>(L);
-    scatter_once(L);
+    nee_continue<FIELD>(L);
 }
 
 // End of a march segment: the face / sensor logic of movePhoton
 // (scene.cpp:719-779) and movePhotonTillSensor (scene.cpp:634-700).
-template <int FIELD, bool RECORD>
+template <int FIELD, int MODE>
 __device__ void seg_end(Lane &L) {
     const DevScene &S = L.S;
     Hot &h = L.h;
@@ -511,6 +579,7 @@ __device__ void seg_end(Lane &L) {
             const float mag = len(h.v);
             const f3 d1 = bsdf_sample(L, h.v / mag, nrm);
             if (dot(d1, nrm) < 0.f) {
+                if constexpr (MODE == M_ADJ) adj_log_main(L, K_FACE, nrm);
                 h.v = d1 * mag;
                 start_segment(L, c.large);
             } else {
@@ -524,6 +593,7 @@ __device__ void seg_end(Lane &L) {
         } else {
             ++c.depth;
         }
+        if constexpr (MODE == M_ADJ) adj_log_main(L, K_DIST, mk3(0.f, 0.f, 0.f));
         vertex(L);
         return;
     }
@@ -543,6 +613,7 @@ __device__ void seg_end(Lane &L) {
             } else {
                 const f3 d1 = bsdf_sample(L, h.v, nrm);  // unnormalised v (scene.cpp:686)
                 if (dot(d1, nrm) < 0.f) {
+                    if constexpr (is_adj(MODE)) adj_log_sub(L, nrm);
                     h.v = d1;
                     start_segment(L, c.large);
                     return;
@@ -552,9 +623,9 @@ __device__ void seg_end(Lane &L) {
         }
     }
     if (c.phase == PH_DIRECT)
-        direct_done<FIELD, RECORD>(L, res);
+        direct_done<FIELD, MODE>(L, res);
     else
-        nee_done<FIELD>(L, res);
+        nee_done<FIELD, MODE>(L, res);
 }
 
 // pmf.h:109-121
@@ -575,12 +646,13 @@ __device__ __forceinline__ int cdf_sample(const DevScene &S, float u) {
 
 // photon.cpp:312-333 up to the direct trace: genRay (scene.cpp:107-137,
 // lens.cpp:19-33), duty-cycle phase, v = n d.
-template <int FIELD, bool RECORD>
+template <int FIELD, int MODE>
 __device__ void photon_init(Lane &L) {
     const DevScene &S = L.S;
     Hot &h = L.h;
     Cold &c = L.c;
     count(L, C_PHOT);
+    if constexpr (is_adj(MODE)) adj_photon_begin(*L.a);
     c.stream = 0;
     c.draw = 0;
     c.faces = 0;
@@ -603,12 +675,12 @@ __device__ void photon_init(Lane &L) {
     pos = pos + dl * dir;
     total += dl;
     if (S.elens_active && !lens_deflect(S.elens_ap2, S.elens_f, pos, dir, total)) {
-        if (RECORD) {
+        if (per_ray(MODE)) {
             h.p = pos;
             h.v = dir;
             h.L = total;
             c.mL = total;
-            direct_done<FIELD, RECORD>(L, -4);
+            direct_done<FIELD, MODE>(L, -4);
         } else {
             photon_end(L);
         }
@@ -620,7 +692,8 @@ __device__ void photon_init(Lane &L) {
     refresh_field<FIELD>(L);
     h.v = h.n0 * dir;
     h.L = total;
-    if (S.use_direct || RECORD) {
+    if (S.use_direct || per_ray(MODE)) {
+        if (L.a) adj_sub_begin(*L.a);
         c.mp = h.p;
         c.mv = h.v;
         c.mL = h.L;
@@ -637,11 +710,15 @@ __device__ void photon_init(Lane &L) {
     }
 }
 
+}  // namespace simpler
+#include "adjoint_kernel.h"
+namespace simpler {
+
 // The persistent kernel.  BATCH: photons of many configurations (shared
 // geometry, per-configuration field / medium / seed / image) in one launch;
 // the global photon counter runs over the concatenated photon budgets and
 // each lane maps its index to (configuration, photon) by binary search.
-template <int FIELD, bool RECORD, bool BATCH>
+template <int FIELD, int MODE, bool BATCH>
 __global__ void __launch_bounds__(kBlock) k_render(DevScene S, RenderArgs A) {
     __shared__ Cold cold[kBlock];
     __shared__ uint32_t cnt[C_N];
@@ -654,7 +731,7 @@ __global__ void __launch_bounds__(kBlock) k_render(DevScene S, RenderArgs A) {
     h.steps = 0;
     h.state = ST_NEED;
     h.dead = false;
-    Lane L{h, cold[threadIdx.x], v, S, A, cnt};
+    Lane L{h, cold[threadIdx.x], v, S, A, cnt, nullptr};
     const int lane = __lane_id();
     while (true) {
         // fetch: one atomic per wave for all lanes that need a photon
@@ -690,8 +767,8 @@ __global__ void __launch_bounds__(kBlock) k_render(DevScene S, RenderArgs A) {
                 }
             }
         }
-        if (h.state == ST_INIT) photon_init<FIELD, RECORD>(L);
-        if (h.state == ST_SEG_END) seg_end<FIELD, RECORD>(L);
+        if (h.state == ST_INIT) photon_init<FIELD, MODE>(L);
+        if (h.state == ST_SEG_END) seg_end<FIELD, MODE>(L);
         if (__ballot(h.state != ST_DONE) == 0ull) break;
         if (__ballot(h.state == ST_MARCH) == 0ull) continue;
         const FieldP F = v.f;
@@ -747,6 +824,9 @@ struct simpler_ctx {
     int num_cu = 256;
     int block = 256, spc = 32, bpc = 4;
     int live = 0;  // scenes and batches not yet destroyed
+    float *d_log = nullptr;  // adjoint segment logs (grown on demand)
+    size_t log_bytes = 0;
+    unsigned long long *d_adj = nullptr;  // adjoint objective (double) + counters
     std::string err;
 };
 
@@ -1013,14 +1093,14 @@ static int check_desc(simpler_ctx *ctx, const simpler_scene_desc *d) {
     return SIMPLER_OK;
 }
 
-template <bool RECORD>
+template <int MODE>
 static void launch_render(simpler_ctx *ctx, int field, const DevScene &S, const RenderArgs &A) {
     const int blocks = std::max(1, ctx->num_cu * ctx->bpc);
     const dim3 grid(blocks), block(ctx->block);
     switch (field) {
-        case SIMPLER_FIELD_BESSEL: hipLaunchKernelGGL((k_render<kBessel, RECORD, false>), grid, block, 0, ctx->cur, S, A); break;
-        case SIMPLER_FIELD_SOURCES: hipLaunchKernelGGL((k_render<kSources, RECORD, false>), grid, block, 0, ctx->cur, S, A); break;
-        default: hipLaunchKernelGGL((k_render<kSpline, RECORD, false>), grid, block, 0, ctx->cur, S, A); break;
+        case SIMPLER_FIELD_BESSEL: hipLaunchKernelGGL((k_render<kBessel, MODE, false>), grid, block, 0, ctx->cur, S, A); break;
+        case SIMPLER_FIELD_SOURCES: hipLaunchKernelGGL((k_render<kSources, MODE, false>), grid, block, 0, ctx->cur, S, A); break;
+        default: hipLaunchKernelGGL((k_render<kSpline, MODE, false>), grid, block, 0, ctx->cur, S, A); break;
     }
 }
 
@@ -1115,6 +1195,8 @@ int simpler_ctx_destroy(simpler_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->cur);
     (void)hipFree(ctx->d_scratch);
+    (void)hipFree(ctx->d_log);
+    (void)hipFree(ctx->d_adj);
     (void)hipStreamDestroy(ctx->own);
     delete ctx;
     return SIMPLER_OK;
@@ -1363,9 +1445,9 @@ static int render_impl(simpler_scene *sc, int64_t n_photons, int64_t n_total, ui
     }
     if (n_photons > 0) {
         if (record)
-            launch_render<true>(ctx, sc->desc.field_kind, S, A);
+            launch_render<M_RECORD>(ctx, sc->desc.field_kind, S, A);
         else
-            launch_render<false>(ctx, sc->desc.field_kind, S, A);
+            launch_render<M_RENDER>(ctx, sc->desc.field_kind, S, A);
     }
     HIP_TRY(ctx, hipGetLastError());
     return SIMPLER_OK;
@@ -1452,6 +1534,135 @@ int simpler_field_eval(simpler_scene *sc, int n_pts, const float *pts_dev, float
     return SIMPLER_OK;
 }
 
+// ---------------------------------------------------------------- adjoint
+constexpr int kAdjMainCap = 512;  // logged main-path segments per photon
+constexpr int kAdjSubCap = 64;    // logged face reflections per NEE / direct sub-path
+constexpr int kAdjBlocksPerCU = 2;
+
+int simpler_adjoint_param_count(const simpler_scene *sc, int64_t *n) {
+    if (!sc || !n) return SIMPLER_ERR_INVALID;
+    switch (sc->desc.field_kind) {
+        case SIMPLER_FIELD_SPLINE: *n = (int64_t)sc->sN[0] * sc->sN[1] * sc->sN[2]; return SIMPLER_OK;
+        case SIMPLER_FIELD_SOURCES: *n = 2 * (int64_t)sc->n_src + 1; return SIMPLER_OK;
+        default: *n = 0; return fail(sc->ctx, SIMPLER_ERR_INVALID, "adjoint: Bessel fields have no differentiable parameters here");
+    }
+}
+
+static int adjoint_impl(simpler_scene *sc, int mode, int64_t n, int64_t n_total, uint64_t seed, uint64_t offset,
+                        const float *G, int X, int Y, int Z, const float *up_p, const float *up_d, const float *up_L,
+                        float *grad, int accumulate, double *obj_host, simpler_adjoint_stats *st) {
+    simpler_ctx *ctx = sc->ctx;
+    int rc = scene_ready(sc);
+    if (rc) return rc;
+    const simpler_scene_desc &d = sc->desc;
+    int64_t npar = 0;
+    rc = simpler_adjoint_param_count(sc, &npar);
+    if (rc) return rc;
+    if (d.field_kind == SIMPLER_FIELD_SOURCES && sc->n_src > kMaxAdjSources)
+        return fail(ctx, SIMPLER_ERR_INVALID, "adjoint: at most 16 point sources");
+    if (d.precision > 7) return fail(ctx, SIMPLER_ERR_INVALID, "adjoint: precision must be <= 7");
+    if (d.sensor_lens_active) return fail(ctx, SIMPLER_ERR_INVALID, "adjoint: the sensor lens is not differentiated");
+    if (!grad) return fail(ctx, SIMPLER_ERR_INVALID, "adjoint: null gradient array");
+    if (n < 0) return fail(ctx, SIMPLER_ERR_INVALID, "adjoint: n must be >= 0");
+    if (mode == M_ADJ) {
+        if (!G) return fail(ctx, SIMPLER_ERR_INVALID, "adjoint: null objective weights");
+        if (X <= 0 || Y <= 0 || Z <= 0 || (double)X * Y * Z > 2147483647.0)
+            return fail(ctx, SIMPLER_ERR_INVALID, "adjoint: bad image resolution");
+        if (n_total > 0 && n_total < n) return fail(ctx, SIMPLER_ERR_INVALID, "n_total must be >= n_photons");
+    } else if (!up_p || !up_d || !up_L) {
+        return fail(ctx, SIMPLER_ERR_INVALID, "adjoint: null per-ray gradient array");
+    }
+    if (n_total <= 0) n_total = n > 0 ? n : 1;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const int blocks = std::max(1, ctx->num_cu * kAdjBlocksPerCU);
+    const size_t lanes = (size_t)blocks * kBlock;
+    const size_t bytes = lanes * (size_t)(kAdjMainCap + kAdjSubCap) * kRecFloats * sizeof(float);
+    if (ctx->log_bytes < bytes) {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
+        (void)hipFree(ctx->d_log);
+        ctx->d_log = nullptr;
+        ctx->log_bytes = 0;
+        HIP_TRY(ctx, hipMalloc(&ctx->d_log, bytes));
+        ctx->log_bytes = bytes;
+    }
+    if (!ctx->d_adj) HIP_TRY(ctx, hipMalloc(&ctx->d_adj, 8 * sizeof(unsigned long long)));
+    const DevScene S = make_dev_scene(sc, n_total, seed, mode == M_ADJRAYS);
+    RenderArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.X = X;
+    A.Y = Y;
+    A.Z = Z;
+    A.offset = offset;
+    A.n = (uint64_t)n;
+    A.work = ctx->d_scratch;
+    A.counters = ctx->d_scratch + 8;
+    A.spc = ctx->spc;
+    A.G = G;
+    A.grad = grad;
+    A.obj = reinterpret_cast<double *>(ctx->d_adj);
+    A.log = ctx->d_log;
+    A.main_cap = kAdjMainCap;
+    A.sub_cap = kAdjSubCap;
+    A.up_p = up_p;
+    A.up_d = up_d;
+    A.up_L = up_L;
+    A.acnt = ctx->d_adj + 1;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, ctx->d_scratch, 16);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, ctx->d_adj, 8);
+    if (!accumulate && npar > 0) {
+        const int fb = (int)std::min<int64_t>((npar + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_fill_f32, dim3(fb), dim3(256), 0, ctx->cur, grad, (size_t)npar, 0.0f);
+    }
+    if (n > 0) {
+        const dim3 grid(blocks), block(kBlock);
+        const bool spline = d.field_kind == SIMPLER_FIELD_SPLINE;
+        if (mode == M_ADJ) {
+            if (spline)
+                hipLaunchKernelGGL((k_adjoint<kSpline, M_ADJ>), grid, block, 0, ctx->cur, S, A);
+            else
+                hipLaunchKernelGGL((k_adjoint<kSources, M_ADJ>), grid, block, 0, ctx->cur, S, A);
+        } else {
+            if (spline)
+                hipLaunchKernelGGL((k_adjoint<kSpline, M_ADJRAYS>), grid, block, 0, ctx->cur, S, A);
+            else
+                hipLaunchKernelGGL((k_adjoint<kSources, M_ADJRAYS>), grid, block, 0, ctx->cur, S, A);
+        }
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    if (obj_host || st) {
+        unsigned long long c[8], a[8];
+        HIP_TRY(ctx, hipMemcpyAsync(c, ctx->d_scratch + 8, sizeof(c), hipMemcpyDeviceToHost, ctx->cur));
+        HIP_TRY(ctx, hipMemcpyAsync(a, ctx->d_adj, sizeof(a), hipMemcpyDeviceToHost, ctx->cur));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
+        if (obj_host) std::memcpy(obj_host, &a[0], sizeof(double));
+        if (st) {
+            st->photons = c[C_PHOT];
+            st->er_steps = c[C_STEPS];
+            st->reverse_steps = a[1 + A_REV_STEPS];
+            st->sub_paths_reversed = a[1 + A_SUB_REV];
+            st->main_paths_reversed = a[1 + A_MAIN_REV];
+            st->dropped = a[1 + A_DROPPED];
+        }
+    }
+    return SIMPLER_OK;
+}
+
+int simpler_adjoint_render(simpler_scene *sc, int64_t n_photons, int64_t n_total, uint64_t seed, uint64_t offset,
+                           const float *dF_dimage, int X, int Y, int Z, float *grad, int accumulate, double *objective,
+                           simpler_adjoint_stats *stats) {
+    if (!sc) return SIMPLER_ERR_INVALID;
+    return adjoint_impl(sc, M_ADJ, n_photons, n_total, seed, offset, dF_dimage, X, Y, Z, nullptr, nullptr, nullptr,
+                        grad, accumulate, objective, stats);
+}
+
+int simpler_adjoint_rays(simpler_scene *sc, int64_t n_rays, uint64_t seed, uint64_t offset, const float *g_pos,
+                         const float *g_dir, const float *g_len, float *grad, int accumulate, double *objective,
+                         simpler_adjoint_stats *stats) {
+    if (!sc) return SIMPLER_ERR_INVALID;
+    return adjoint_impl(sc, M_ADJRAYS, n_rays, n_rays, seed, offset, nullptr, 1, 1, 1, g_pos, g_dir, g_len, grad,
+                        accumulate, objective, stats);
+}
+
 }  // extern "C"
 
 struct simpler_batch {
@@ -1492,7 +1703,7 @@ static int batch_launch(simpler_batch *b, float *images, int accumulate, hipStre
     }
     if (b->total > 0) {
         const int blocks = std::max(1, ctx->num_cu * ctx->bpc);
-        hipLaunchKernelGGL((k_render<kBessel, false, true>), dim3(blocks), dim3(ctx->block), 0, stream, b->S, A);
+        hipLaunchKernelGGL((k_render<kBessel, M_RENDER, true>), dim3(blocks), dim3(ctx->block), 0, stream, b->S, A);
     }
     return SIMPLER_OK;
 }

@@ -11,7 +11,7 @@ import weakref
 import numpy as np
 
 from . import build as _build
-from ._abi import BatchCfg, SceneDesc, Stats
+from ._abi import AdjointStats, BatchCfg, SceneDesc, Stats
 
 _LIB = None
 
@@ -64,12 +64,18 @@ def load_library():
     L.simpler_batch_graph_capture.argtypes = [P, P]
     L.simpler_batch_graph_launch.argtypes = [P]
     L.simpler_batch_destroy.argtypes = [P]
+    L.simpler_adjoint_param_count.argtypes = [P, C.POINTER(C.c_int64)]
+    L.simpler_adjoint_render.argtypes = [P, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, P, C.c_int, C.c_int,
+                                         C.c_int, P, C.c_int, C.POINTER(C.c_double), C.POINTER(AdjointStats)]
+    L.simpler_adjoint_rays.argtypes = [P, C.c_int64, C.c_uint64, C.c_uint64, P, P, P, P, C.c_int,
+                                       C.POINTER(C.c_double), C.POINTER(AdjointStats)]
     for name in ("simpler_ctx_create", "simpler_ctx_destroy", "simpler_ctx_set_stream", "simpler_ctx_synchronize",
                  "simpler_ctx_set_tuning", "simpler_ctx_get_tuning", "simpler_scene_create", "simpler_scene_destroy",
                  "simpler_scene_update", "simpler_scene_get_desc", "simpler_scene_set_sources",
                  "simpler_scene_set_spline", "simpler_render", "simpler_render_host", "simpler_trace_rays",
                  "simpler_field_eval", "simpler_read_stats", "simpler_batch_create", "simpler_batch_render",
-                 "simpler_batch_graph_capture", "simpler_batch_graph_launch", "simpler_batch_destroy"):
+                 "simpler_batch_graph_capture", "simpler_batch_graph_launch", "simpler_batch_destroy",
+                 "simpler_adjoint_param_count", "simpler_adjoint_render", "simpler_adjoint_rays"):
         getattr(L, name).restype = C.c_int
     _LIB = L
     return L
@@ -172,11 +178,43 @@ class Scene:
         coeff = np.ascontiguousarray(coeff, np.float32)
         self.ctx.check(self.L.simpler_scene_set_sources(self.h, len(amp), _fp(pos), _fp(amp), _fp(phase), len(coeff),
                                                         _fp(coeff)))
+        self._sources = dict(pos=pos, amp=amp, phase=phase, coeff=coeff)
 
     def set_spline(self, coeffs, dims, xmin, xmax):
         c = np.ascontiguousarray(coeffs, np.float32).reshape(-1)
         self.ctx.check(self.L.simpler_scene_set_spline(self.h, _fp(c), (C.c_int32 * 3)(*dims),
                                                        (C.c_float * 3)(*xmin), (C.c_float * 3)(*xmax)))
+        self._spline = (tuple(dims), tuple(xmin), tuple(xmax), c)
+
+    def params(self):
+        """The differentiable field parameters as a float64 vector: spline
+        coefficients, or source amplitudes, phases and the wavenumber
+        k = 2 pi f_u / speed_u."""
+        if self.desc.field_kind == 2:
+            return self._spline[3].astype(np.float64)
+        if self.desc.field_kind == 1:
+            s = self._sources
+            k = 2.0 * np.pi * float(self.desc.f_u) / float(self.desc.speed_u)
+            return np.concatenate([s["amp"], s["phase"], [k]]).astype(np.float64)
+        raise SimplerError("Bessel fields have no differentiable parameters")
+
+    def set_params(self, vec):
+        """Inverse of params()."""
+        vec = np.asarray(vec, np.float64).reshape(-1)
+        if self.desc.field_kind == 2:
+            dims, lo, hi, _ = self._spline
+            self.set_spline(vec, dims, lo, hi)
+        elif self.desc.field_kind == 1:
+            s = self._sources
+            N = len(s["amp"])
+            if vec.size != 2 * N + 1:
+                raise SimplerError(f"expected {2 * N + 1} source parameters")
+            self.set_sources(s["pos"], vec[:N], vec[N:2 * N], s["coeff"])
+            d = self.desc.copy()
+            d.set(f_u=float(vec[2 * N]) * float(d.speed_u) / (2.0 * np.pi))
+            self.update(d)
+        else:
+            raise SimplerError("Bessel fields have no differentiable parameters")
 
     def update(self, desc):
         self.desc = desc.copy()
@@ -217,6 +255,48 @@ class Scene:
                                                  C.c_void_p(d.data_ptr()), C.c_void_p(L.data_ptr()),
                                                  C.c_void_p(s.data_ptr())))
         return p, d, L, s
+
+    def param_count(self):
+        """Number of differentiable field parameters (spline coefficients, or
+        source amplitudes + phases + wavenumber)."""
+        n = C.c_int64()
+        self.ctx.check(self.L.simpler_adjoint_param_count(self.h, C.byref(n)))
+        return n.value
+
+    def _grad_buffer(self, grad):
+        import torch
+        n = self.param_count()
+        if grad is None:
+            return torch.empty((n,), dtype=torch.float32, device=f"cuda:{self.ctx.device}"), False
+        if grad.dtype != torch.float32 or not grad.is_contiguous() or grad.numel() != n:
+            raise SimplerError(f"gradient buffer must be a contiguous float32 tensor of {n} elements")
+        return grad, True
+
+    def adjoint_render(self, dF_dimage, n, seed=1, offset=0, n_total=None, grad=None):
+        """Path-replay adjoint of F = sum(dF_dimage * render(n, ...)).
+        dF_dimage: CUDA float32 tensor (Z, Y, X).  Returns (gradient tensor,
+        F, stats); with grad given, the gradient is added to it."""
+        G = dF_dimage.contiguous()
+        Z, Y, X = G.shape
+        g, acc = self._grad_buffer(grad)
+        F = C.c_double()
+        st = AdjointStats()
+        self.ctx.check(self.L.simpler_adjoint_render(self.h, n, n_total or n, seed, offset, C.c_void_p(G.data_ptr()),
+                                                     X, Y, Z, C.c_void_p(g.data_ptr()), int(acc), C.byref(F),
+                                                     C.byref(st)))
+        return g, F.value, st.as_dict()
+
+    def adjoint_rays(self, g_pos, g_dir, g_len, seed=1, offset=0, grad=None):
+        """Adjoint of F = sum g_pos . exit_pos + g_dir . exit_dir + g_len * len
+        over the rays of trace_rays(n, seed, offset), n = len(g_len)."""
+        gp, gd, gl = g_pos.contiguous(), g_dir.contiguous(), g_len.contiguous()
+        g, acc = self._grad_buffer(grad)
+        F = C.c_double()
+        st = AdjointStats()
+        self.ctx.check(self.L.simpler_adjoint_rays(self.h, gl.numel(), seed, offset, C.c_void_p(gp.data_ptr()),
+                                                   C.c_void_p(gd.data_ptr()), C.c_void_p(gl.data_ptr()),
+                                                   C.c_void_p(g.data_ptr()), int(acc), C.byref(F), C.byref(st)))
+        return g, F.value, st.as_dict()
 
     def field(self, pts, scaling=1.0, tau=0.0):
         """n and grad n at points (CUDA tensor (N, 3) float32)."""

@@ -179,6 +179,37 @@ def spline_cube(n=64, sigma_s=2000.0, g=0.9, step=5e-6, res=(64, 64, 32), seed=0
     return d, disk_texture(501), res, (vals, dims, lo, hi)
 
 
+def prefilter3d(vals, dims):
+    """B-spline coefficients of node values (x fastest, dims (N0, N1, N2))
+    with the engine's host prefilter (scene_pybind.prefilter3d)."""
+    import sys
+    from . import build as _build
+    if _build.LIB not in sys.path:
+        sys.path.insert(0, _build.LIB)
+    import scene_pybind
+    N0, N1, N2 = dims
+    a = np.ascontiguousarray(np.asarray(vals, np.float32).reshape(N2, N1, N0))
+    return np.asarray(scene_pybind.prefilter3d(a), np.float32).reshape(-1)
+
+
+def spline_cube_ready(n=64, sigma_s=2000.0, g=0.9, step=5e-6, res=(64, 64, 32), seed=0):
+    """spline_cube with the node values already prefiltered into
+    coefficients: (desc, texture, res, (coeffs, dims, xmin, xmax))."""
+    d, tex, res, (vals, dims, lo, hi) = spline_cube(n=n, sigma_s=sigma_s, g=g, step=step, res=res, seed=seed)
+    return d, tex, res, (prefilter3d(vals, dims), dims, lo, hi)
+
+
+def aperture_weights(res, view_plane=0.010, diameter=0.001):
+    """dF/dimage of the throughput into a centred circular aperture on the
+    sensor: 1 on pixels whose centre lies inside, all time bins; (Z, Y, X)."""
+    X, Y, Z = res
+    px = view_plane / X
+    yy, xx = np.mgrid[0:Y, 0:X]
+    r2 = ((xx + 0.5 - X / 2) * px) ** 2 + ((yy + 0.5 - Y / 2) * px) ** 2
+    m = (r2 <= (diameter / 2) ** 2).astype(np.float32)
+    return np.repeat(m[None], Z, axis=0)
+
+
 def splitmix64(x):
     """SplitMix64 finaliser (Steele, Lea, Flood 2014): seed derivation."""
     x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF

@@ -26,3 +26,16 @@ def gpu_ctx():
 def gpu_threads():
     """Worker threads for the oracle in GPU tests (box CPU share)."""
     return min(16, os.cpu_count() or 1)
+
+
+# GPU test files run in dependency order -- forward parity first, then the
+# layers built on it -- so that under -x a failure names the lowest layer.
+_GPU_ORDER = ["test_gpu_parity.py", "test_gpu_host_layer.py", "test_gpu_batch.py", "test_gpu_adjoint.py"]
+
+
+def pytest_collection_modifyitems(config, items):
+    def rank(item):
+        name = os.path.basename(str(item.fspath))
+        return _GPU_ORDER.index(name) if name in _GPU_ORDER else -1
+
+    items.sort(key=rank)

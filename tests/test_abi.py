@@ -9,7 +9,7 @@ import tempfile
 import pytest
 
 from simpler_amd import build
-from simpler_amd._abi import BatchCfg, SceneDesc, Stats
+from simpler_amd._abi import AdjointStats, BatchCfg, SceneDesc, Stats
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "simpler.h")
@@ -45,6 +45,7 @@ def _c_layout():
     fields_desc = [f for f, _ in SceneDesc._fields_]
     fields_stats = [f for f, _ in Stats._fields_]
     fields_batch = [f for f, _ in BatchCfg._fields_]
+    fields_adj = [f for f, _ in AdjointStats._fields_]
     prog = ['#include <stdio.h>', '#include <stddef.h>', '#include "simpler.h"', 'int main(void){',
             'printf("%zu %zu\\n", sizeof(simpler_scene_desc), sizeof(simpler_stats));']
     for f in fields_desc:
@@ -54,6 +55,9 @@ def _c_layout():
     prog.append('printf("%zu\\n", sizeof(simpler_batch_cfg));')
     for f in fields_batch:
         prog.append(f'printf("%zu\\n", offsetof(simpler_batch_cfg, {f}));')
+    prog.append('printf("%zu\\n", sizeof(simpler_adjoint_stats));')
+    for f in fields_adj:
+        prog.append(f'printf("%zu\\n", offsetof(simpler_adjoint_stats, {f}));')
     prog.append("return 0;}")
     with tempfile.TemporaryDirectory() as td:
         c = os.path.join(td, "p.c")
@@ -62,19 +66,22 @@ def _c_layout():
         subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
         out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
     vals = [int(x) for x in out]
-    nd, ns = len(fields_desc), len(fields_stats)
-    return (vals[0], vals[1], vals[2:2 + nd], vals[2 + nd:2 + nd + ns], vals[2 + nd + ns],
-            vals[3 + nd + ns:])
+    nd, ns, nb = len(fields_desc), len(fields_stats), len(fields_batch)
+    o = 2 + nd + ns
+    return (vals[0], vals[1], vals[2:2 + nd], vals[2 + nd:o], vals[o], vals[o + 1:o + 1 + nb],
+            vals[o + 1 + nb], vals[o + 2 + nb:])
 
 
 def test_ctypes_mirror_matches_c_layout():
-    sd, ss, off_d, off_s, sb, off_b = _c_layout()
+    sd, ss, off_d, off_s, sb, off_b, sa, off_a = _c_layout()
     assert C.sizeof(SceneDesc) == sd
     assert C.sizeof(Stats) == ss
     assert C.sizeof(BatchCfg) == sb
     assert [getattr(SceneDesc, f).offset for f, _ in SceneDesc._fields_] == off_d
     assert [getattr(Stats, f).offset for f, _ in Stats._fields_] == off_s
     assert [getattr(BatchCfg, f).offset for f, _ in BatchCfg._fields_] == off_b
+    assert C.sizeof(AdjointStats) == sa
+    assert [getattr(AdjointStats, f).offset for f, _ in AdjointStats._fields_] == off_a
 
 
 def test_default_descriptor_and_version_without_gpu():
