@@ -46,6 +46,7 @@ def gpu_bench(args):
     from simpler_amd import engine, scenes
 
     ctx = engine.Context(0)
+    autotuned = None
     if args.tune:
         ctx.set_tuning(*[int(x) for x in args.tune.split(",")])
     B = args.photons_per_step
@@ -57,6 +58,9 @@ def gpu_bench(args):
         X, Y, Z = res
         img = torch.zeros((Z, Y, X), dtype=torch.float32, device="cuda")
         work[m] = (sc, img, res)
+    if not args.tune and args.autotune_photons > 0:  # the autotuner picks the launch shape
+        autotuned = work[modes[0]][0].autotune(args.autotune_photons, work[modes[0]][2])
+        eprint(f"autotune: {autotuned}")
     stream = torch.cuda.current_stream()
 
     def step(k, record):
@@ -105,7 +109,7 @@ def gpu_bench(args):
                                       er_steps_per_photon=pm["er_steps"] / pm["photons"],
                                       tflops=pm["flops"] / (pm["kernel_ms"] / 1e3) / 1e12)
                          for m, pm in per_mode.items()},
-               tuning=ctx.get_tuning(), launches=len(recs))
+               tuning=dict(ctx.get_tuning(), autotuned=autotuned is not None), launches=len(recs))
     ctx.close()
     return res
 
@@ -241,6 +245,18 @@ def cpu_baseline(args):
                                            note="same CPU code with the dead-path cutoff the GPU uses"))
 
 
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """A plain device copy of known size, for scaling rocprof's FETCH_SIZE /
+    WRITE_SIZE counters to bytes (run under rocprofv3 --pmc)."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,11 +264,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons-per-step", type=int, default=1 << 22)
     ap.add_argument("--modes", default="0,2")
-    ap.add_argument("--tune", default="", help="block_size,steps_per_chunk,blocks_per_cu")
+    ap.add_argument("--tune", default="", help="block_size,steps_per_chunk,blocks_per_cu (skips the autotuner)")
+    ap.add_argument("--autotune-photons", type=int, default=1 << 19, help="0: keep the default launch shape")
     ap.add_argument("--cpu-photons", type=int, default=96)
     ap.add_argument("--cpu-photons-cutoff", type=int, default=768)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (for scaling rocprof byte counters)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-configs", type=int, default=256)
     ap.add_argument("--batch-scale", type=float, default=0.02,
@@ -263,6 +282,9 @@ def main():
     ap.add_argument("--adjoint-fd", type=int, default=8, help="coefficients checked by central differences")
     ap.add_argument("--adjoint-eps", type=float, default=5e-5)
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for {max(args.gpus, world)}); photon shards would be "

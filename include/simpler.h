@@ -159,10 +159,13 @@ int simpler_ctx_create(int device, simpler_ctx **out);
  * alive (destroy them first); scene_destroy likewise fails while batches
  * built on the scene are alive. */
 int simpler_ctx_destroy(simpler_ctx *ctx);
-/* Use an external HIP stream (hipStream_t passed as void*); NULL restores the
- * context's own stream. */
+/* Run the context's work on a caller's HIP stream (hipStream_t passed as
+ * void*).  NULL is the HIP null stream, which orders with every blocking
+ * stream (e.g. PyTorch's default stream); a new context uses its own
+ * non-blocking stream, whose handle simpler_ctx_own_stream returns. */
 int simpler_ctx_set_stream(simpler_ctx *ctx, void *hip_stream);
 void *simpler_ctx_get_stream(simpler_ctx *ctx);
+void *simpler_ctx_own_stream(simpler_ctx *ctx);
 int simpler_ctx_synchronize(simpler_ctx *ctx);
 const char *simpler_last_error(simpler_ctx *ctx);
 
@@ -340,6 +343,16 @@ int simpler_ctx_set_tuning(simpler_ctx *ctx, int block_size,
                            int steps_per_chunk, int blocks_per_cu);
 int simpler_ctx_get_tuning(simpler_ctx *ctx, int *block_size,
                            int *steps_per_chunk, int *blocks_per_cu);
+
+/* Autotuner (no reference counterpart: the reference hard-codes its launch,
+ * cuda_renderer.cu:763-767): times an n_photons render of the scene at the
+ * given resolution for every candidate (block size 128/256, steps per chunk
+ * 16/32/64, blocks per CU 3/4/5), keeps the fastest on the scene's context
+ * and returns it (outputs may be NULL).  The splat has a single strategy
+ * (per-tap device atomics), so it is not a candidate.  Synchronous. */
+int simpler_autotune(simpler_scene *scene, int64_t n_photons, int x_res,
+                     int y_res, int z_res, int *block_size, int *steps_per_chunk,
+                     int *blocks_per_cu, float *ms_best);
 
 #ifdef __cplusplus
 }

@@ -161,8 +161,9 @@ __device__ __forceinline__ f3 reflect_vec(f3 x, f3 nrm) { return x - (2.f * dot(
 //             lp += h/2 H(p) lvh;  lv = lvh  (start kick; at p: a = -s, b = h/2 lvh)
 // The start point is p = q - h vh / n(p), solved with n evaluated at
 // q - h vh / n(q) (the error is O(h^3 |grad n|^2), far below FP32).
+// Returns false when the segment start is reached.
 template <int FIELD>
-__device__ __forceinline__ void rev_step(Hot &h, AdjState &a, const DevScene &S, const FieldP &F) {
+__device__ __forceinline__ bool rev_step(Hot &h, AdjState &a, const DevScene &S, const FieldP &F) {
     const float hs = a.rh;
     const float hh = 0.5f * hs;
     a.lp = a.lp + hh * symmul(a.H, a.lv);
@@ -192,7 +193,7 @@ __device__ __forceinline__ void rev_step(Hot &h, AdjState &a, const DevScene &S,
     h.g0 = g1;
     a.H = H1;
     ++a.rsteps;
-    h.state = rev_next(a, S.h0) ? ST_RMARCH : ST_REND;
+    return rev_next(a, S.h0);
 }
 
 // Henyey-Greenstein scatter v_out = |v_in| T(v_in / |v_in|) with

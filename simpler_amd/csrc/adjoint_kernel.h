@@ -158,12 +158,17 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
         if (__ballot(h.state == ST_MARCH || h.state == ST_RMARCH) == 0ull) continue;
         const FieldP F = v.f;
         const float Lcut = v.Lcut;
+        bool m = h.state == ST_MARCH, r = h.state == ST_RMARCH;
+        uint32_t sl = h.steps_left;
         for (int k = 0; k < A.spc; ++k) {
-            if (h.state == ST_MARCH)
-                march_step<FIELD, true>(h, S, F, Lcut);
-            else if (h.state == ST_RMARCH)
-                rev_step<FIELD>(h, a, S, F);
+            if (m)
+                m = !march_step<FIELD, true>(h, sl, S, F, Lcut);
+            else if (r)
+                r = rev_step<FIELD>(h, a, S, F);
         }
+        h.steps_left = sl;
+        if (h.state == ST_MARCH && !m) h.state = ST_SEG_END;
+        if (h.state == ST_RMARCH && !r) h.state = ST_REND;
     }
     unsigned long long st = h.steps;
     double fo = a.F;

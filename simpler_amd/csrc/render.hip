@@ -45,6 +45,7 @@ __host__ __device__ constexpr bool is_adj(int m) { return m == M_ADJ || m == M_A
 enum { C_PHOT = 0, C_STEPS, C_SCAT, C_NEE, C_DIRECT, C_FACE, C_SEG, C_ABORT, C_N };
 constexpr uint32_t kMaxFaceEvents = 1u << 20;  // per march; see the abort counter
 constexpr int kBlock = 256;
+constexpr int kJtabLds = 128;  // Bessel table intervals staged in LDS (8 KiB)
 
 struct RenderArgs {
     float *image;
@@ -74,6 +75,7 @@ struct RenderArgs {
 // VGPRs.
 struct Hot {
     f3 p, pl, v, g0;  // position (+ compensation), v = n d, grad n at p
+    f3 dv;            // velocity change since the segment start (v + dv is the velocity)
     float n0, L, Ll;  // n at p, optical length (+ compensation)
     float travelled, seg_dist, h, scaling, tau;
     int prec;
@@ -158,15 +160,17 @@ __device__ __forceinline__ void start_segment(Lane &L, float dist) {
     h.state = ST_MARCH;
     h.fcnt0 = 0;
     h.fpat = 0;
+    h.dv = mk3(0.f, 0.f, 0.f);
     count(L, C_SEG);
 }
 
 __device__ __forceinline__ void end_segment(Hot &h) {
     h.p = h.p + h.pl;
     h.pl = mk3(0.f, 0.f, 0.f);
+    h.v = h.v + h.dv;
+    h.dv = mk3(0.f, 0.f, 0.f);
     h.L += h.Ll;
     h.Ll = 0.f;
-    h.state = ST_SEG_END;
 }
 
 __device__ __forceinline__ bool inside_block(const DevScene &S, f3 p) {
@@ -177,14 +181,21 @@ __device__ __forceinline__ bool inside_block(const DevScene &S, f3 p) {
 // One ER step of traceTillBlock (scene.cpp:395-418) with er_step
 // (scene.cpp:323-330): kick (h/2) grad n(p), drift h v / n(p), kick
 // (h/2) grad n(p').  (n, grad n) at the start point is the previous step's
-// end-point evaluation.
+// end-point evaluation.  The kicks accumulate into dv, the change since the
+// segment start, which stays small: a kick of (h/2)|grad n| ~ 1e-7 is then
+// kept to FP32 precision instead of being rounded against |v| ~ 1.33.
 // LOG: also record the step pattern (accepted steps per refinement level,
 // final level, kept overshoot step) that the adjoint replays backwards.
+// steps_left is the caller's register copy of h.steps_left; returns true when
+// the segment ended (the caller then sets ST_SEG_END), so the loop state
+// never goes through memory.
 template <int FIELD, bool LOG = false>
-__device__ __forceinline__ void march_step(Hot &h, const DevScene &S, const FieldP &F, float Lcut) {
+__device__ __forceinline__ bool march_step(Hot &h, uint32_t &steps_left, const DevScene &S, const FieldP &F,
+                                           float Lcut) {
     const float hs = h.h;
     const float hh = 0.5f * hs;
-    const f3 vh = h.v + hh * h.g0;
+    const f3 dvh = h.dv + hh * h.g0;
+    const f3 vh = h.v + dvh;
     const float hn = hs * __builtin_amdgcn_rcpf(h.n0);
     float px = h.p.x, py = h.p.y, pz = h.p.z;
     float lx = h.pl.x, ly = h.pl.y, lz = h.pl.z;
@@ -195,7 +206,7 @@ __device__ __forceinline__ void march_step(Hot &h, const DevScene &S, const Fiel
     float n1;
     f3 g1;
     field_eval<FIELD>(S, F, p1, h.scaling, h.tau, n1, g1);
-    const f3 v1 = vh + hh * g1;
+    const f3 dv1 = dvh + hh * g1;
     ++h.steps;
     const int lev = S.precision - h.prec;
     const bool reject = !inside_block(S, p1) || (h.travelled + hs) > h.seg_dist;
@@ -203,17 +214,21 @@ __device__ __forceinline__ void march_step(Hot &h, const DevScene &S, const Fiel
     if (!reject || stop) {
         h.p = p1;
         h.pl = mk3(lx, ly, lz);
-        h.v = v1;
+        h.dv = dv1;
         h.n0 = n1;
         h.g0 = g1;
     }
     if (stop) {
         if (LOG) h.fpat |= ((uint32_t)lev << 28) | (1u << 31);
         end_segment(h);
-    } else if (reject) {  // restore and refine: h/10, 10 more steps (scene.cpp:408-412)
+        return true;
+    }
+    if (reject) {  // restore and refine: h/10, 10 more steps (scene.cpp:408-412)
         h.h = hs / 10.0f;
-        h.steps_left = 10;
-    } else {
+        steps_left = 10;
+        return false;
+    }
+    {
         h.travelled += hs;
         comp_add(h.L, h.Ll, hs * n1);  // optical length (scene.cpp:416)
         if (LOG) {
@@ -225,11 +240,15 @@ __device__ __forceinline__ void march_step(Hot &h, const DevScene &S, const Fiel
         if (h.L > Lcut) {
             h.dead = true;
             end_segment(h);
-        } else if (--h.steps_left == 0) {
+            return true;
+        }
+        if (--steps_left == 0) {
             if (LOG) h.fpat |= (uint32_t)lev << 28;
             end_segment(h);
+            return true;
         }
     }
+    return false;
 }
 
 // scene.cpp:655-676: outward normal of the nearest face (strict <, order
@@ -718,14 +737,22 @@ namespace simpler {
 // geometry, per-configuration field / medium / seed / image) in one launch;
 // the global photon counter runs over the concatenated photon budgets and
 // each lane maps its index to (configuration, photon) by binary search.
-template <int FIELD, int MODE, bool BATCH>
+//
+// JLDS (single-scene Bessel fields with a table of at most kJtabLds
+// intervals): the J_m / J_m' table is staged in LDS once per workgroup, so the
+// per-step lookup is an LDS read instead of an L1/L2 round trip.
+template <int FIELD, int MODE, bool BATCH, bool JLDS = false>
 __global__ void __launch_bounds__(kBlock) k_render(DevScene S, RenderArgs A) {
     __shared__ Cold cold[kBlock];
     __shared__ uint32_t cnt[C_N];
     __shared__ Var vars[BATCH ? kBlock : 1];
+    __shared__ float4 jlds[JLDS ? 4 * kJtabLds : 1];
     if (threadIdx.x < C_N) cnt[threadIdx.x] = 0u;
+    if (JLDS)
+        for (int i = threadIdx.x; i < 4 * S.jtab_n; i += blockDim.x) jlds[i] = S.jtab[i];
     __syncthreads();
     Var vsingle = var_from(S, A);
+    if (JLDS) vsingle.f.jtab = jlds;
     Var &v = BATCH ? vars[threadIdx.x] : vsingle;
     Hot h;
     h.steps = 0;
@@ -773,9 +800,12 @@ __global__ void __launch_bounds__(kBlock) k_render(DevScene S, RenderArgs A) {
         if (__ballot(h.state == ST_MARCH) == 0ull) continue;
         const FieldP F = v.f;
         const float Lcut = v.Lcut;
-        for (int k = 0; k < A.spc; ++k) {
-            if (h.state == ST_MARCH) march_step<FIELD>(h, S, F, Lcut);
-        }
+        bool m = h.state == ST_MARCH;
+        uint32_t sl = h.steps_left;
+        for (int k = 0; k < A.spc; ++k)
+            if (m) m = !march_step<FIELD>(h, sl, S, F, Lcut);
+        h.steps_left = sl;
+        if (h.state == ST_MARCH && !m) h.state = ST_SEG_END;
     }
     // counters: ER steps reduced over the wave, events from LDS, one atomic
     // per counter per wave / block
@@ -1098,7 +1128,12 @@ static void launch_render(simpler_ctx *ctx, int field, const DevScene &S, const 
     const int blocks = std::max(1, ctx->num_cu * ctx->bpc);
     const dim3 grid(blocks), block(ctx->block);
     switch (field) {
-        case SIMPLER_FIELD_BESSEL: hipLaunchKernelGGL((k_render<kBessel, MODE, false>), grid, block, 0, ctx->cur, S, A); break;
+        case SIMPLER_FIELD_BESSEL:
+            if (S.jtab_n <= kJtabLds)
+                hipLaunchKernelGGL((k_render<kBessel, MODE, false, true>), grid, block, 0, ctx->cur, S, A);
+            else
+                hipLaunchKernelGGL((k_render<kBessel, MODE, false>), grid, block, 0, ctx->cur, S, A);
+            break;
         case SIMPLER_FIELD_SOURCES: hipLaunchKernelGGL((k_render<kSources, MODE, false>), grid, block, 0, ctx->cur, S, A); break;
         default: hipLaunchKernelGGL((k_render<kSpline, MODE, false>), grid, block, 0, ctx->cur, S, A); break;
     }
@@ -1204,11 +1239,12 @@ int simpler_ctx_destroy(simpler_ctx *ctx) {
 
 int simpler_ctx_set_stream(simpler_ctx *ctx, void *stream) {
     if (!ctx) return SIMPLER_ERR_INVALID;
-    ctx->cur = stream ? (hipStream_t)stream : ctx->own;
+    ctx->cur = (hipStream_t)stream;
     return SIMPLER_OK;
 }
 
 void *simpler_ctx_get_stream(simpler_ctx *ctx) { return ctx ? (void *)ctx->cur : nullptr; }
+void *simpler_ctx_own_stream(simpler_ctx *ctx) { return ctx ? (void *)ctx->own : nullptr; }
 
 int simpler_ctx_synchronize(simpler_ctx *ctx) {
     if (!ctx) return SIMPLER_ERR_INVALID;
@@ -1234,6 +1270,71 @@ int simpler_ctx_set_tuning(simpler_ctx *ctx, int block_size, int steps_per_chunk
         if (blocks_per_cu < 1 || blocks_per_cu > 64) return fail(ctx, SIMPLER_ERR_INVALID, "blocks_per_cu out of range [1,64]");
         ctx->bpc = blocks_per_cu;
     }
+    return SIMPLER_OK;
+}
+
+static int render_impl(simpler_scene *sc, int64_t n_photons, int64_t n_total, uint64_t seed, uint64_t offset,
+                       float *image, int X, int Y, int Z, int accumulate, bool record, float *rp, float *rd, float *rL,
+                       int32_t *rs);
+
+int simpler_autotune(simpler_scene *sc, int64_t n_photons, int X, int Y, int Z, int *block_out, int *spc_out,
+                     int *bpc_out, float *ms_out) {
+    if (!sc) return SIMPLER_ERR_INVALID;
+    simpler_ctx *ctx = sc->ctx;
+    if (n_photons <= 0 || X <= 0 || Y <= 0 || Z <= 0 || (double)X * Y * Z > 2147483647.0)
+        return fail(ctx, SIMPLER_ERR_INVALID, "autotune: bad photon count or resolution");
+    int rc = scene_ready(sc);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    float *img = nullptr;
+    HIP_TRY(ctx, hipMalloc(&img, (size_t)X * Y * Z * sizeof(float)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        (void)hipFree(img);
+        return fail(ctx, SIMPLER_ERR_HIP, "autotune: event creation failed");
+    }
+    const int blocks[] = {128, 256}, spcs[] = {16, 32, 64}, bpcs[] = {3, 4, 5};
+    int best[3] = {ctx->block, ctx->spc, ctx->bpc};
+    float best_ms = INFINITY;
+    rc = SIMPLER_OK;
+    bool warm = false;
+    for (int b : blocks)
+        for (int spc : spcs)
+            for (int bpc : bpcs) {
+                if (rc) break;
+                ctx->block = b;
+                ctx->spc = spc;
+                ctx->bpc = bpc;
+                if (!warm) {  // first launch of the kernel: load the code object
+                    rc = render_impl(sc, n_photons / 8 + 1, n_photons, 1, 0, img, X, Y, Z, 0, false, nullptr, nullptr,
+                                     nullptr, nullptr);
+                    warm = true;
+                }
+                if (!rc && hipEventRecord(e0, ctx->cur) != hipSuccess) rc = fail(ctx, SIMPLER_ERR_HIP, "autotune: event");
+                if (!rc)
+                    rc = render_impl(sc, n_photons, n_photons, 1, 0, img, X, Y, Z, 0, false, nullptr, nullptr, nullptr,
+                                     nullptr);
+                if (!rc && (hipEventRecord(e1, ctx->cur) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+                    rc = fail(ctx, SIMPLER_ERR_HIP, "autotune: render failed");
+                float ms = 0.f;
+                if (!rc && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best_ms) {
+                    best_ms = ms;
+                    best[0] = b;
+                    best[1] = spc;
+                    best[2] = bpc;
+                }
+            }
+    ctx->block = best[0];
+    ctx->spc = best[1];
+    ctx->bpc = best[2];
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(img);
+    if (rc) return rc;
+    if (block_out) *block_out = best[0];
+    if (spc_out) *spc_out = best[1];
+    if (bpc_out) *bpc_out = best[2];
+    if (ms_out) *ms_out = best_ms;
     return SIMPLER_OK;
 }
 

@@ -41,6 +41,8 @@ def load_library():
     L.simpler_ctx_set_stream.argtypes = [P, P]
     L.simpler_ctx_get_stream.argtypes = [P]
     L.simpler_ctx_get_stream.restype = P
+    L.simpler_ctx_own_stream.argtypes = [P]
+    L.simpler_ctx_own_stream.restype = P
     L.simpler_ctx_synchronize.argtypes = [P]
     L.simpler_last_error.argtypes = [P]
     L.simpler_last_error.restype = C.c_char_p
@@ -64,6 +66,8 @@ def load_library():
     L.simpler_batch_graph_capture.argtypes = [P, P]
     L.simpler_batch_graph_launch.argtypes = [P]
     L.simpler_batch_destroy.argtypes = [P]
+    L.simpler_autotune.argtypes = [P, C.c_int64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int), C.POINTER(C.c_float)]
     L.simpler_adjoint_param_count.argtypes = [P, C.POINTER(C.c_int64)]
     L.simpler_adjoint_render.argtypes = [P, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, P, C.c_int, C.c_int,
                                          C.c_int, P, C.c_int, C.POINTER(C.c_double), C.POINTER(AdjointStats)]
@@ -75,7 +79,8 @@ def load_library():
                  "simpler_scene_set_spline", "simpler_render", "simpler_render_host", "simpler_trace_rays",
                  "simpler_field_eval", "simpler_read_stats", "simpler_batch_create", "simpler_batch_render",
                  "simpler_batch_graph_capture", "simpler_batch_graph_launch", "simpler_batch_destroy",
-                 "simpler_adjoint_param_count", "simpler_adjoint_render", "simpler_adjoint_rays"):
+                 "simpler_adjoint_param_count", "simpler_adjoint_render", "simpler_adjoint_rays",
+                 "simpler_autotune"):
         getattr(L, name).restype = C.c_int
     _LIB = L
     return L
@@ -255,6 +260,15 @@ class Scene:
                                                  C.c_void_p(d.data_ptr()), C.c_void_p(L.data_ptr()),
                                                  C.c_void_p(s.data_ptr())))
         return p, d, L, s
+
+    def autotune(self, n, res):
+        """Pick block size, steps per chunk and blocks per CU for this scene
+        by timing an n-photon render of each candidate; the choice stays on
+        the context.  Returns it with the best time (ms)."""
+        b, s, c, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+        X, Y, Z = res
+        self.ctx.check(self.L.simpler_autotune(self.h, n, X, Y, Z, C.byref(b), C.byref(s), C.byref(c), C.byref(ms)))
+        return dict(block_size=b.value, steps_per_chunk=s.value, blocks_per_cu=c.value, ms=ms.value)
 
     def param_count(self):
         """Number of differentiable field parameters (spline coefficients, or
