@@ -30,6 +30,11 @@ N_TOTAL = 10 ** 9
 F_STEP = {0: 92, 1: 126, 2: 148}
 F_EVENT = dict(scatters=80, nee_splats=127, direct_splats=100, face_events=45, photons=40)
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per photon of k_render (mode 0), from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE of one 2^20-photon launch (3,938.3 and 1,307,812.0 KiB),
+# scaled by the counters' reading for a 1 GiB device copy (x2.00 fetch,
+# x1.00 write): almost all of it is the splat atomics.
+TRAFFIC_BYTES_PER_PHOTON = 1284.85
 
 
 def eprint(*a):
@@ -58,8 +63,9 @@ def gpu_bench(args):
         X, Y, Z = res
         img = torch.zeros((Z, Y, X), dtype=torch.float32, device="cuda")
         work[m] = (sc, img, res)
-    if not args.tune and args.autotune_photons > 0:  # the autotuner picks the launch shape
-        autotuned = work[modes[0]][0].autotune(args.autotune_photons, work[modes[0]][2])
+    n_tune = args.photons_per_step if args.autotune_photons < 0 else args.autotune_photons
+    if not args.tune and n_tune > 0:  # the autotuner picks the launch shape on the workload itself
+        autotuned = work[modes[0]][0].autotune(n_tune, work[modes[0]][2])
         eprint(f"autotune: {autotuned}")
     stream = torch.cuda.current_stream()
 
@@ -155,9 +161,10 @@ def adjoint_bench(args):
     seeded noise, Gaussian-filtered sigma = 4 cells, max |dn| 5e-5), sigma_s
     2/mm, g 0.9, 5 um ER step, 64x64x32 sensor.  Objective: throughput into a
     1 mm aperture.  Times the forward render and the path-replay adjoint
-    (gradient w.r.t. all 262,144 coefficients) on the same photons, and checks
-    the adjoint against central differences with common random numbers on K
-    coefficients drawn at random among the top 10 % by gradient magnitude."""
+    (gradient w.r.t. all 262,144 coefficients) on the same photons.  Checks
+    the adjoint against central differences of the GPU forward with common
+    random numbers     on K coefficients: the K/2 with the largest gradient and K/2 drawn at
+    random from the top 10 %."""
     import numpy as np
     import torch
 
@@ -178,6 +185,11 @@ def adjoint_bench(args):
         torch.cuda.synchronize()
         return out, e0.elapsed_time(e1) / 1e3
 
+    def objective(c):
+        sc.set_spline(c, dims, lo, hi)
+        (im, _), t = timed(lambda: sc.render(N, res, seed=11, stats=False))
+        return float((im.double() * G.double()).sum()), t
+
     sc.render(4096, res, seed=11)
     sc.adjoint_render(G, 4096, seed=11)
     (img, fst), t_fwd = timed(lambda: sc.render(N, res, seed=11))
@@ -185,20 +197,20 @@ def adjoint_bench(args):
     (grad, F, ast), t_adj = timed(lambda: sc.adjoint_render(G, N, seed=11))
     grad = grad.cpu().numpy().astype(np.float64)
     eprint(f"adjoint: {N} paths, forward {t_fwd:.2f}s, adjoint {t_adj:.2f}s, F {F:.6g} (render {F_img:.6g})")
+    t_fd = 0.0
     rng = np.random.default_rng(3)
-    top = np.argsort(-np.abs(grad))[: max(args.adjoint_fd, len(grad) // 10)]
-    picks = rng.choice(top, args.adjoint_fd, replace=False)
-    errs, t_fd = [], 0.0
+    order = np.argsort(-np.abs(grad))
+    n_top = args.adjoint_fd // 2
+    picks = np.concatenate([order[:n_top], rng.choice(order[n_top:len(grad) // 10], args.adjoint_fd - n_top,
+                                                      replace=False)])
+    errs = []
     for j in picks:
-        Fs = []
-        for sgn in (1, -1):
-            c = coeff.copy()
-            c[j] = np.float32(coeff[j] + sgn * args.adjoint_eps)
-            sc.set_spline(c, dims, lo, hi)
-            (im, _), t = timed(lambda: sc.render(N, res, seed=11, stats=False))
-            t_fd += t
-            Fs.append((float(c[j]), float((im.double() * G.double()).sum())))
-        fd = (Fs[0][1] - Fs[1][1]) / (Fs[0][0] - Fs[1][0])
+        c1, c2 = coeff.copy(), coeff.copy()
+        c1[j] = np.float32(coeff[j] + args.adjoint_eps)
+        c2[j] = np.float32(coeff[j] - args.adjoint_eps)
+        (F1, ta), (F2, tb) = objective(c1), objective(c2)
+        t_fd += ta + tb
+        fd = (F1 - F2) / (float(c1[j]) - float(c2[j]))
         errs.append(abs(grad[j] - fd) / abs(fd) if fd != 0 else float("inf"))
         eprint(f"  coefficient {j}: adjoint {grad[j]:.5g}  central difference {fd:.5g}  rel err {errs[-1]:.3g}")
     sc.set_spline(coeff, dims, lo, hi)
@@ -208,9 +220,9 @@ def adjoint_bench(args):
                          "throughput into a 1 mm aperture"),
                paths=N, parameters=n_par, forward_s=t_fwd, adjoint_s=t_adj, adjoint_over_forward=t_adj / t_fwd,
                objective=F, objective_render=F_img, forward_paths_per_s=N / t_fwd,
-               fd_check=dict(coefficients=[int(j) for j in picks], rel_err=errs,
-                             median_rel_err=float(np.median(errs)), max_rel_err=float(np.max(errs)),
-                             eps=args.adjoint_eps, seconds_per_coefficient=per_fd),
+               fd_check=dict(coefficients=[int(j) for j in picks], coefficient_rel_err=errs,
+                             max_rel_err=float(max(errs)), median_rel_err=float(np.median(errs)),
+                             coefficient_eps=args.adjoint_eps, seconds_per_gradient_entry=per_fd),
                fd_full_gradient_s=per_fd * n_par, adjoint_speedup_over_fd=per_fd * n_par / t_adj,
                adjoint_stats=ast, forward_er_steps=fst["er_steps"])
     sc.close()
@@ -265,7 +277,8 @@ def main():
     ap.add_argument("--photons-per-step", type=int, default=1 << 22)
     ap.add_argument("--modes", default="0,2")
     ap.add_argument("--tune", default="", help="block_size,steps_per_chunk,blocks_per_cu (skips the autotuner)")
-    ap.add_argument("--autotune-photons", type=int, default=1 << 19, help="0: keep the default launch shape")
+    ap.add_argument("--autotune-photons", type=int, default=-1,
+                    help="photons per autotuner trial (default: the step's own chunk size; 0: no autotuning)")
     ap.add_argument("--cpu-photons", type=int, default=96)
     ap.add_argument("--cpu-photons-cutoff", type=int, default=768)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -278,9 +291,9 @@ def main():
                     help="photon budgets of the batched sweep times this factor (1.0: 5.5e8 photons)")
     ap.add_argument("--batch-replays", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--adjoint-photons", type=int, default=1 << 20)
+    ap.add_argument("--adjoint-photons", type=int, default=1 << 19)
     ap.add_argument("--adjoint-fd", type=int, default=8, help="coefficients checked by central differences")
-    ap.add_argument("--adjoint-eps", type=float, default=5e-5)
+    ap.add_argument("--adjoint-eps", type=float, default=1e-4)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -319,7 +332,8 @@ def main():
         "per_mode": g["per_mode"],
         "tuning": g["tuning"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": TRAFFIC_BYTES_PER_PHOTON * args.photons_per_step,
                      "kernel": "k_render<Bessel> (persistent march + event kernel)",
                      "avg_launch_ms": g["kernel_ms"] / g["launches"]},
     }
