@@ -279,6 +279,7 @@ typedef struct simpler_batch_cfg {
 } simpler_batch_cfg;
 
 typedef struct simpler_batch simpler_batch;
+struct simpler_adjoint_stats; /* defined with the adjoint entry points below */
 
 int simpler_batch_create(simpler_scene *base, int n_cfg, const simpler_batch_cfg *cfgs_host, int x_res,
                          int y_res, int z_res, simpler_batch **out);
@@ -289,6 +290,27 @@ int simpler_batch_render(simpler_batch *batch, float *images_dev, int accumulate
 int simpler_batch_graph_capture(simpler_batch *batch, float *images_dev);
 int simpler_batch_graph_launch(simpler_batch *batch);
 int simpler_batch_destroy(simpler_batch *batch);
+
+/* ---- point-source pattern batch (no reference counterpart: the reference
+ *      sweeps its phases one renderImage call at a time, e.g.
+ *      FUS_configurations.ipynb with set_phase1/set_phase2, scene.h:179-219).
+ *      The base scene (a point-source field with its sources) fixes source
+ *      positions, harmonic weights, k, geometry, medium and flags; pattern c
+ *      has its own amplitudes amp[c*n_src + i], phases phase[c*n_src + i],
+ *      photon budget and seed.  Render with simpler_batch_render (images c =
+ *      pattern c) or capture it as a graph like any batch. -------------------- */
+int simpler_batch_create_sources(simpler_scene *base, int n_cfg, const float *amp_host,
+                                 const float *phase_host, const int64_t *n_photons_host,
+                                 const uint64_t *seeds_host, int x_res, int y_res, int z_res,
+                                 simpler_batch **out);
+
+/* Path-replay adjoint of every pattern of a point-source batch in one launch:
+ * F_c = sum dF_dimages[c] * image_c; grads_dev[c * (2 n_src + 1) + j] receives
+ * dF_c / dtheta_j in the layout of simpler_adjoint_param_count (amplitudes,
+ * phases, k); objectives_host (n_cfg doubles) and stats_host may be NULL. */
+int simpler_batch_adjoint(simpler_batch *batch, const float *dF_dimages_dev,
+                          float *grads_dev, int accumulate, double *objectives_host,
+                          struct simpler_adjoint_stats *stats_host);
 
 /* ---- path-replay adjoint (no counterpart in the reference, which has no
  *      gradient with respect to field parameters; its forward-mode

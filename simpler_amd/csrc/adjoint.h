@@ -372,7 +372,7 @@ __device__ bool nee_adj_start(Lane &L, f3 ps, float Lv, float val, f3 din, f3 dt
     const DevScene &S = L.S;
     float Wsum;
     f3 lps;
-    if (!splat_adj(S, L.A, L.A.G, ps, Lv, L.c.depth, val, Wsum, lps)) return false;
+    if (!splat_adj(S, L.A, L.v.image, ps, Lv, L.c.depth, val, Wsum, lps)) return false;
     a.F += (double)val * (double)Wsum;
     if (a.trunc_sub || a.trunc_main) {
         ++a.dropped;
@@ -403,7 +403,7 @@ __device__ bool direct_adj_start(Lane &L, f3 ps, float Lv, float val) {
     const DevScene &S = L.S;
     float Wsum;
     f3 lps;
-    if (!splat_adj(S, L.A, L.A.G, ps, Lv, 0, val, Wsum, lps)) return false;
+    if (!splat_adj(S, L.A, L.v.image, ps, Lv, 0, val, Wsum, lps)) return false;
     a.F += (double)val * (double)Wsum;
     if (a.trunc_sub) {
         ++a.dropped;

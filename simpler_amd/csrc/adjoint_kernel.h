@@ -101,18 +101,38 @@ __device__ void rev_end(Lane &L) {
     }
 }
 
+// A lane's source-parameter accumulators and objective, added to the
+// configuration's gradient and objective (batched adjoint: when the lane moves
+// to another configuration, and at the end).
+__device__ __forceinline__ void adj_flush_cfg(AdjState &a, const RenderArgs &A, int cfg, int npar) {
+    if (cfg < 0) return;
+    for (int j = 0; j < npar; ++j) {
+        const float x = a.G.acc[j * a.G.stride];
+        if (x != 0.f) atomicAdd(A.grad + (size_t)cfg * npar + j, x);
+        a.G.acc[j * a.G.stride] = 0.f;
+    }
+    if (a.F != 0.0) atomicAdd(A.obj + cfg, a.F);
+    a.F = 0.0;
+}
+
 // The persistent adjoint kernel: the render kernel's fetch / events / march
 // loop plus the backward events (ST_REND, ST_MREV) and backward steps.
-template <int FIELD, int MODE>
+// BATCH: photons of many point-source patterns (per-pattern amplitudes,
+// phases, photon budget, seed, objective weights, gradient and objective).
+template <int FIELD, int MODE, bool BATCH = false>
 __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
     __shared__ Cold cold[kBlock];
     __shared__ uint32_t cnt[C_N];
     __shared__ float sacc[FIELD == kSources ? (2 * kMaxAdjSources + 1) * kBlock : 1];
+    __shared__ Var vars[BATCH ? kBlock : 1];
     if (threadIdx.x < C_N) cnt[threadIdx.x] = 0u;
     const int npar = FIELD == kSources ? 2 * S.n_src + 1 : 0;
     for (int j = 0; j < npar; ++j) sacc[j * kBlock + threadIdx.x] = 0.f;
     __syncthreads();
-    Var v = var_from(S, A);
+    Var vsingle = var_from(S, A);
+    vsingle.image = const_cast<float *>(A.G);  // the objective weights, read by the splat adjoint
+    Var &v = BATCH ? vars[threadIdx.x] : vsingle;
+    int cur_cfg = -1;
     Hot h;
     h.steps = 0;
     h.state = ST_NEED;
@@ -141,7 +161,24 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
                 const unsigned long long rank = __popcll(need & ((1ull << lane) - 1ull));
                 const unsigned long long idx = base + rank;
                 if (idx < A.n) {
-                    const uint64_t pid = A.offset + idx;
+                    uint64_t pid = A.offset + idx;
+                    if (BATCH) {
+                        int lo = 0, hi = A.n_cfg;  // largest c with prefix[c] <= idx
+                        while (hi - lo > 1) {
+                            const int mid = (lo + hi) >> 1;
+                            if (A.prefix[mid] <= idx)
+                                lo = mid;
+                            else
+                                hi = mid;
+                        }
+                        pid = idx - A.prefix[lo];
+                        if (lo != cur_cfg) {
+                            adj_flush_cfg(a, A, cur_cfg, npar);
+                            cur_cfg = lo;
+                            v = A.cfgs[lo];
+                            v.image = const_cast<float *>(A.G) + (size_t)lo * A.X * A.Y * A.Z;
+                        }
+                    }
                     L.c.pid_lo = (uint32_t)pid;
                     L.c.pid_hi = (uint32_t)(pid >> 32);
                     h.state = ST_INIT;
@@ -170,6 +207,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
         if (h.state == ST_MARCH && !m) h.state = ST_SEG_END;
         if (h.state == ST_RMARCH && !r) h.state = ST_REND;
     }
+    if (BATCH) adj_flush_cfg(a, A, cur_cfg, npar);
     unsigned long long st = h.steps;
     double fo = a.F;
     unsigned long long ac[A_N] = {a.rsteps, a.nsubrev, a.nmainrev, a.dropped};
@@ -189,7 +227,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
     __syncthreads();
     if (threadIdx.x < C_N && threadIdx.x != C_STEPS && cnt[threadIdx.x])
         atomicAdd(A.counters + threadIdx.x, (unsigned long long)cnt[threadIdx.x]);
-    for (int j = threadIdx.x; j < npar; j += kBlock) {
+    for (int j = threadIdx.x; j < (BATCH ? 0 : npar); j += kBlock) {
         float sum = 0.f;
         for (int t = 0; t < kBlock; ++t) sum += sacc[j * kBlock + t];
         if (sum != 0.f) atomicAdd(A.grad + j, sum);

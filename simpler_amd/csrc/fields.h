@@ -69,7 +69,9 @@ enum FieldKind { kBessel = 0, kSources = 1, kSpline = 2 };
 struct FieldP {
     float amp, k_r;
     int mode, jtab_n;
-    const float4 *jtab;
+    const float4 *jtab;   // Bessel J_m / J_m' table
+    const float4 *src;    // point sources (x, y, z, amplitude)
+    const float *phase;   // point-source phases
 };
 
 __device__ __forceinline__ FieldP field_params(const DevScene &S) {
@@ -79,6 +81,8 @@ __device__ __forceinline__ FieldP field_params(const DevScene &S) {
     F.mode = S.mode;
     F.jtab_n = S.jtab_n;
     F.jtab = S.jtab;
+    F.src = S.src;
+    F.phase = S.src_phase;
     return F;
 }
 
@@ -173,12 +177,12 @@ __device__ __forceinline__ void field_eval<kSources>(const DevScene &S, const Fi
     f3 gacc = mk3(0.f, 0.f, 0.f);
     const float tshift = S.time_coupling * tau;
     for (int i = 0; i < S.n_src; ++i) {
-        const float4 s = S.src[i];
+        const float4 s = F.src[i];
         const f3 dv = mk3(p.x - s.x, p.y - s.y, p.z - s.z);
         const float r2 = dot(dv, dv);
         const float ir = rsqrtf(r2);
         const float r = r2 * ir;
-        const float ph0 = S.src_phase[i] - tshift;
+        const float ph0 = F.phase[i] - tshift;
         float vs = 0.f, gs = 0.f;
         for (int c = 1; c <= S.n_coeff; ++c) {
             const float krc = F.k_r * r * (float)c;
@@ -330,12 +334,12 @@ __device__ __forceinline__ void field_eval_h<kSources>(const DevScene &S, const 
     f3 gacc = mk3(0.f, 0.f, 0.f);
     const float tshift = S.time_coupling * tau;
     for (int i = 0; i < S.n_src; ++i) {
-        const float4 s = S.src[i];
+        const float4 s = F.src[i];
         const f3 dv = mk3(p.x - s.x, p.y - s.y, p.z - s.z);
         const float r2 = dot(dv, dv);
         const float ir = rsqrtf(r2);
         const float r = r2 * ir;
-        const float ph0 = S.src_phase[i] - tshift;
+        const float ph0 = F.phase[i] - tshift;
         float vs = 0.f, gs = 0.f, hs = 0.f;
         for (int c = 1; c <= S.n_coeff; ++c) {
             const float krc = F.k_r * r * (float)c;
@@ -375,12 +379,12 @@ __device__ __forceinline__ void field_vjp<kSources>(const DevScene &S, const Fie
     const int N = S.n_src;
     float gk = 0.f;
     for (int i = 0; i < N; ++i) {
-        const float4 s = S.src[i];
+        const float4 s = F.src[i];
         const f3 dv = mk3(p.x - s.x, p.y - s.y, p.z - s.z);
         const float r2 = dot(dv, dv);
         const float ir = rsqrtf(r2);
         const float r = r2 * ir;
-        const float ph0 = S.src_phase[i] - tshift;
+        const float ph0 = F.phase[i] - tshift;
         float vs = 0.f, gs = 0.f, ss = 0.f, kc = 0.f, cs1 = 0.f, c2c = 0.f;
         for (int c = 1; c <= S.n_coeff; ++c) {
             const float fc = (float)c;

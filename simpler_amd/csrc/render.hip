@@ -1652,6 +1652,23 @@ int simpler_adjoint_param_count(const simpler_scene *sc, int64_t *n) {
     }
 }
 
+// Segment logs (one list per lane of the adjoint grid) and the objective /
+// counter words, allocated on first use and kept by the context.
+static int adjoint_buffers(simpler_ctx *ctx) {
+    const size_t lanes = (size_t)std::max(1, ctx->num_cu * kAdjBlocksPerCU) * kBlock;
+    const size_t bytes = lanes * (size_t)(kAdjMainCap + kAdjSubCap) * kRecFloats * sizeof(float);
+    if (ctx->log_bytes < bytes) {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
+        (void)hipFree(ctx->d_log);
+        ctx->d_log = nullptr;
+        ctx->log_bytes = 0;
+        HIP_TRY(ctx, hipMalloc(&ctx->d_log, bytes));
+        ctx->log_bytes = bytes;
+    }
+    if (!ctx->d_adj) HIP_TRY(ctx, hipMalloc(&ctx->d_adj, 8 * sizeof(unsigned long long)));
+    return SIMPLER_OK;
+}
+
 static int adjoint_impl(simpler_scene *sc, int mode, int64_t n, int64_t n_total, uint64_t seed, uint64_t offset,
                         const float *G, int X, int Y, int Z, const float *up_p, const float *up_d, const float *up_L,
                         float *grad, int accumulate, double *obj_host, simpler_adjoint_stats *st) {
@@ -1679,17 +1696,8 @@ static int adjoint_impl(simpler_scene *sc, int mode, int64_t n, int64_t n_total,
     if (n_total <= 0) n_total = n > 0 ? n : 1;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     const int blocks = std::max(1, ctx->num_cu * kAdjBlocksPerCU);
-    const size_t lanes = (size_t)blocks * kBlock;
-    const size_t bytes = lanes * (size_t)(kAdjMainCap + kAdjSubCap) * kRecFloats * sizeof(float);
-    if (ctx->log_bytes < bytes) {
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
-        (void)hipFree(ctx->d_log);
-        ctx->d_log = nullptr;
-        ctx->log_bytes = 0;
-        HIP_TRY(ctx, hipMalloc(&ctx->d_log, bytes));
-        ctx->log_bytes = bytes;
-    }
-    if (!ctx->d_adj) HIP_TRY(ctx, hipMalloc(&ctx->d_adj, 8 * sizeof(unsigned long long)));
+    rc = adjoint_buffers(ctx);
+    if (rc) return rc;
     const DevScene S = make_dev_scene(sc, n_total, seed, mode == M_ADJRAYS);
     RenderArgs A;
     std::memset(&A, 0, sizeof(A));
@@ -1777,6 +1785,9 @@ struct simpler_batch {
     Var *d_cfgs = nullptr;
     unsigned long long *d_prefix = nullptr;
     float4 *d_jtab = nullptr;
+    int field = SIMPLER_FIELD_BESSEL;
+    float4 *d_src = nullptr;  // point-source patterns: n_cfg x n_src (x, y, z, amplitude)
+    float *d_phase = nullptr;  // and their phases
     DevScene S;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1807,7 +1818,12 @@ static int batch_launch(simpler_batch *b, float *images, int accumulate, hipStre
     }
     if (b->total > 0) {
         const int blocks = std::max(1, ctx->num_cu * ctx->bpc);
-        hipLaunchKernelGGL((k_render<kBessel, M_RENDER, true>), dim3(blocks), dim3(ctx->block), 0, stream, b->S, A);
+        if (b->field == SIMPLER_FIELD_SOURCES)
+            hipLaunchKernelGGL((k_render<kSources, M_RENDER, true>), dim3(blocks), dim3(ctx->block), 0, stream, b->S,
+                               A);
+        else
+            hipLaunchKernelGGL((k_render<kBessel, M_RENDER, true>), dim3(blocks), dim3(ctx->block), 0, stream, b->S,
+                               A);
     }
     return SIMPLER_OK;
 }
@@ -1856,6 +1872,8 @@ int simpler_batch_create(simpler_scene *base, int n_cfg, const simpler_batch_cfg
         v.f.mode = Si.mode;
         v.f.jtab_n = nint;
         v.f.jtab = nullptr;  // patched below
+        v.f.src = nullptr;
+        v.f.phase = nullptr;
         v.n_max_scaling = Si.n_max_scaling;
         v.phi_min = Si.phi_min;
         v.phi_range = Si.phi_range;
@@ -1899,6 +1917,161 @@ int simpler_batch_create(simpler_scene *base, int n_cfg, const simpler_batch_cfg
     b->counted = true;
     *out = b;
     return SIMPLER_OK;
+}
+
+int simpler_batch_create_sources(simpler_scene *base, int n_cfg, const float *amp, const float *phase,
+                                 const int64_t *n_photons, const uint64_t *seeds, int X, int Y, int Z,
+                                 simpler_batch **out) {
+    if (!base || !out) return SIMPLER_ERR_INVALID;
+    *out = nullptr;
+    simpler_ctx *ctx = base->ctx;
+    if (base->desc.field_kind != SIMPLER_FIELD_SOURCES || base->n_src <= 0)
+        return fail(ctx, SIMPLER_ERR_INVALID, "pattern batch: the base scene must be a point-source field with sources");
+    if (n_cfg <= 0 || !amp || !phase || !n_photons || !seeds)
+        return fail(ctx, SIMPLER_ERR_INVALID, "pattern batch: need n_cfg > 0 and all arrays");
+    if (X <= 0 || Y <= 0 || Z <= 0 || (double)n_cfg * X * Y * Z > 4.0e9)
+        return fail(ctx, SIMPLER_ERR_INVALID, "pattern batch: bad image resolution");
+    const int N = base->n_src;
+    std::vector<Var> vars(n_cfg);
+    std::vector<unsigned long long> prefix(n_cfg + 1, 0ull);
+    std::vector<float4> src((size_t)n_cfg * N);
+    for (int i = 0; i < n_cfg; ++i) {
+        if (n_photons[i] < 0) return fail(ctx, SIMPLER_ERR_INVALID, "pattern batch: n_photons must be >= 0");
+        simpler_scene tmp = *base;  // shallow: descriptor and host source copy only
+        for (int k = 0; k < N; ++k) {
+            const float a = amp[(size_t)i * N + k];
+            if (!std::isfinite(a) || !std::isfinite(phase[(size_t)i * N + k]))
+                return fail(ctx, SIMPLER_ERR_INVALID, "pattern batch: non-finite amplitude or phase");
+            tmp.h_src[4 * k + 3] = a;  // dead-path cutoff bound of this pattern
+            src[(size_t)i * N + k] = make_float4(base->h_src[4 * k], base->h_src[4 * k + 1], base->h_src[4 * k + 2], a);
+        }
+        const DevScene Si = make_dev_scene(&tmp, n_photons[i] > 0 ? n_photons[i] : 1, seeds[i], false);
+        Var &v = vars[i];
+        v.f = FieldP{Si.amp, Si.k_r, Si.mode, 0, nullptr, nullptr, nullptr};  // src / phase patched below
+        v.n_max_scaling = Si.n_max_scaling;
+        v.phi_min = Si.phi_min;
+        v.phi_range = Si.phi_range;
+        v.sigma_t = Si.sigma_t;
+        v.mfp = Si.mfp;
+        v.albedo = Si.albedo;
+        v.g = Si.g;
+        v.weight = Si.weight;
+        v.Lcut = Si.Lcut;
+        v.seed_lo = Si.seed_lo;
+        v.seed_hi = Si.seed_hi;
+        v.image = nullptr;
+        prefix[i + 1] = prefix[i] + (unsigned long long)n_photons[i];
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    simpler_batch *b = new simpler_batch();
+    b->ctx = ctx;
+    b->base = base;
+    b->n_cfg = n_cfg;
+    b->X = X;
+    b->Y = Y;
+    b->Z = Z;
+    b->total = prefix[n_cfg];
+    b->field = SIMPLER_FIELD_SOURCES;
+    if (hipMalloc(&b->d_src, src.size() * sizeof(float4)) != hipSuccess ||
+        hipMalloc(&b->d_phase, src.size() * sizeof(float)) != hipSuccess ||
+        hipMalloc(&b->d_cfgs, vars.size() * sizeof(Var)) != hipSuccess ||
+        hipMalloc(&b->d_prefix, prefix.size() * sizeof(unsigned long long)) != hipSuccess) {
+        simpler_batch_destroy(b);
+        return fail(ctx, SIMPLER_ERR_NOMEM, "pattern batch: device allocation failed");
+    }
+    for (int i = 0; i < n_cfg; ++i) {
+        vars[i].f.src = b->d_src + (size_t)i * N;
+        vars[i].f.phase = b->d_phase + (size_t)i * N;
+    }
+    if (hipMemcpy(b->d_src, src.data(), src.size() * sizeof(float4), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_phase, phase, src.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_cfgs, vars.data(), vars.size() * sizeof(Var), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_prefix, prefix.data(), prefix.size() * sizeof(unsigned long long), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+        simpler_batch_destroy(b);
+        return fail(ctx, SIMPLER_ERR_HIP, "pattern batch: upload failed");
+    }
+    b->S = make_dev_scene(base, 1, 0, false);
+    ++ctx->live;
+    ++base->batches;
+    b->counted = true;
+    *out = b;
+    return SIMPLER_OK;
+}
+
+int simpler_batch_adjoint(simpler_batch *b, const float *dF_dimages, float *grads, int accumulate,
+                          double *objectives_host, simpler_adjoint_stats *st) {
+    if (!b) return SIMPLER_ERR_INVALID;
+    simpler_ctx *ctx = b->ctx;
+    simpler_scene *sc = b->base;
+    if (b->field != SIMPLER_FIELD_SOURCES)
+        return fail(ctx, SIMPLER_ERR_INVALID, "batch adjoint: only point-source pattern batches are differentiable");
+    if (!dF_dimages || !grads) return fail(ctx, SIMPLER_ERR_INVALID, "batch adjoint: null array");
+    if (sc->n_src > kMaxAdjSources) return fail(ctx, SIMPLER_ERR_INVALID, "adjoint: at most 16 point sources");
+    if (sc->desc.precision > 7) return fail(ctx, SIMPLER_ERR_INVALID, "adjoint: precision must be <= 7");
+    if (sc->desc.sensor_lens_active)
+        return fail(ctx, SIMPLER_ERR_INVALID, "adjoint: the sensor lens is not differentiated");
+    const int npar = 2 * sc->n_src + 1;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int rc = adjoint_buffers(ctx);
+    if (rc) return rc;
+    double *d_obj = nullptr;
+    HIP_TRY(ctx, hipMalloc(&d_obj, (size_t)b->n_cfg * sizeof(double)));
+    RenderArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.X = b->X;
+    A.Y = b->Y;
+    A.Z = b->Z;
+    A.offset = 0;
+    A.n = b->total;
+    A.work = ctx->d_scratch;
+    A.counters = ctx->d_scratch + 8;
+    A.spc = ctx->spc;
+    A.cfgs = b->d_cfgs;
+    A.prefix = b->d_prefix;
+    A.n_cfg = b->n_cfg;
+    A.G = dF_dimages;
+    A.grad = grads;
+    A.obj = d_obj;
+    A.log = ctx->d_log;
+    A.main_cap = kAdjMainCap;
+    A.sub_cap = kAdjSubCap;
+    A.acnt = ctx->d_adj + 1;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, ctx->d_scratch, 16);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, ctx->d_adj, 8);
+    hipLaunchKernelGGL(k_fill_f32, dim3((b->n_cfg * 2 + 255) / 256), dim3(256), 0, ctx->cur, (float *)d_obj,
+                       (size_t)b->n_cfg * 2, 0.0f);
+    if (!accumulate) {
+        const size_t n = (size_t)b->n_cfg * npar;
+        hipLaunchKernelGGL(k_fill_f32, dim3((int)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0, ctx->cur,
+                           grads, n, 0.0f);
+    }
+    if (b->total > 0) {
+        const int blocks = std::max(1, ctx->num_cu * kAdjBlocksPerCU);
+        hipLaunchKernelGGL((k_adjoint<kSources, M_ADJ, true>), dim3(blocks), dim3(kBlock), 0, ctx->cur, b->S, A);
+    }
+    rc = SIMPLER_OK;
+    if (hipGetLastError() != hipSuccess) rc = fail(ctx, SIMPLER_ERR_HIP, "batch adjoint: launch failed");
+    if (!rc && (objectives_host || st)) {
+        unsigned long long c[8], a[8];
+        if (hipMemcpyAsync(c, ctx->d_scratch + 8, sizeof(c), hipMemcpyDeviceToHost, ctx->cur) != hipSuccess ||
+            hipMemcpyAsync(a, ctx->d_adj, sizeof(a), hipMemcpyDeviceToHost, ctx->cur) != hipSuccess ||
+            (objectives_host && hipMemcpyAsync(objectives_host, d_obj, (size_t)b->n_cfg * sizeof(double),
+                                               hipMemcpyDeviceToHost, ctx->cur) != hipSuccess) ||
+            hipStreamSynchronize(ctx->cur) != hipSuccess) {
+            rc = fail(ctx, SIMPLER_ERR_HIP, "batch adjoint: readback failed");
+        } else if (st) {
+            st->photons = c[C_PHOT];
+            st->er_steps = c[C_STEPS];
+            st->reverse_steps = a[1 + A_REV_STEPS];
+            st->sub_paths_reversed = a[1 + A_SUB_REV];
+            st->main_paths_reversed = a[1 + A_MAIN_REV];
+            st->dropped = a[1 + A_DROPPED];
+        }
+    }
+    (void)hipStreamSynchronize(ctx->cur);
+    (void)hipFree(d_obj);
+    return rc;
 }
 
 int simpler_batch_render(simpler_batch *b, float *images, int accumulate, simpler_stats *stats) {
@@ -1955,6 +2128,8 @@ int simpler_batch_destroy(simpler_batch *b) {
     (void)hipFree(b->d_cfgs);
     (void)hipFree(b->d_prefix);
     (void)hipFree(b->d_jtab);
+    (void)hipFree(b->d_src);
+    (void)hipFree(b->d_phase);
     delete b;
     return SIMPLER_OK;
 }

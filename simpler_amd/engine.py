@@ -69,6 +69,9 @@ def load_library():
     L.simpler_autotune.argtypes = [P, C.c_int64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int), C.POINTER(C.c_float)]
     L.simpler_adjoint_param_count.argtypes = [P, C.POINTER(C.c_int64)]
+    L.simpler_batch_create_sources.argtypes = [P, C.c_int, fptr, fptr, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
+                                               C.c_int, C.c_int, C.c_int, PP]
+    L.simpler_batch_adjoint.argtypes = [P, P, P, C.c_int, C.POINTER(C.c_double), C.POINTER(AdjointStats)]
     L.simpler_adjoint_render.argtypes = [P, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, P, C.c_int, C.c_int,
                                          C.c_int, P, C.c_int, C.POINTER(C.c_double), C.POINTER(AdjointStats)]
     L.simpler_adjoint_rays.argtypes = [P, C.c_int64, C.c_uint64, C.c_uint64, P, P, P, P, C.c_int,
@@ -80,7 +83,7 @@ def load_library():
                  "simpler_field_eval", "simpler_read_stats", "simpler_batch_create", "simpler_batch_render",
                  "simpler_batch_graph_capture", "simpler_batch_graph_launch", "simpler_batch_destroy",
                  "simpler_adjoint_param_count", "simpler_adjoint_render", "simpler_adjoint_rays",
-                 "simpler_autotune"):
+                 "simpler_autotune", "simpler_batch_create_sources", "simpler_batch_adjoint"):
         getattr(L, name).restype = C.c_int
     _LIB = L
     return L
@@ -362,6 +365,48 @@ class Batch:
         self.ctx._batches.add(self)
         self.images = torch.zeros((len(configs), Z, Y, X), dtype=torch.float32, device=f"cuda:{self.ctx.device}")
         self.captured = False
+
+    @classmethod
+    def patterns(cls, scene, amps, phases, n_photons, seeds, res):
+        """Point-source pattern batch: per-pattern amplitudes and phases
+        (arrays (n_cfg, n_src)), photon budgets and seeds over one
+        point-source base scene."""
+        import torch
+        self = cls.__new__(cls)
+        self.scene, self.ctx, self.L, self.res = scene, scene.ctx, scene.L, res
+        amps = np.ascontiguousarray(amps, np.float32)
+        phases = np.ascontiguousarray(phases, np.float32)
+        n_cfg = amps.shape[0]
+        npho = (C.c_int64 * n_cfg)(*[int(x) for x in n_photons])
+        sds = (C.c_uint64 * n_cfg)(*[int(x) for x in seeds])
+        self.n_cfg = n_cfg
+        self.photons = int(sum(int(x) for x in n_photons))
+        self.n_src = amps.shape[1]
+        h = C.c_void_p()
+        X, Y, Z = res
+        self.ctx.check(self.L.simpler_batch_create_sources(scene.h, n_cfg, _fp(amps), _fp(phases), npho, sds, X, Y,
+                                                           Z, C.byref(h)))
+        self.h = h
+        scene._batches.add(self)
+        self.ctx._batches.add(self)
+        self.images = torch.zeros((n_cfg, Z, Y, X), dtype=torch.float32, device=f"cuda:{self.ctx.device}")
+        self.captured = False
+        return self
+
+    def adjoint(self, dF_dimages, grads=None):
+        """Adjoint of every pattern: F_c = sum(dF_dimages[c] * image_c).
+        Returns (gradients (n_cfg, 2 n_src + 1), F (n_cfg,), stats)."""
+        import torch
+        G = dF_dimages.contiguous()
+        npar = 2 * self.n_src + 1
+        acc = grads is not None
+        if grads is None:
+            grads = torch.empty((self.n_cfg, npar), dtype=torch.float32, device=G.device)
+        F = (C.c_double * self.n_cfg)()
+        st = AdjointStats()
+        self.ctx.check(self.L.simpler_batch_adjoint(self.h, C.c_void_p(G.data_ptr()), C.c_void_p(grads.data_ptr()),
+                                                    int(acc), F, C.byref(st)))
+        return grads, np.array(F[:]), st.as_dict()
 
     def render(self, accumulate=False, stats=True):
         st = Stats()

@@ -80,3 +80,40 @@ def test_destroy_order_is_enforced():
     assert ctx.L.simpler_scene_destroy(sc.h) == 3  # batch alive
     ctx.close()  # closes the batch, then the scene, then the context
     assert b.h is None and sc.h is None and ctx.h is None
+
+
+def pattern_setup(n_cfg=4, seed=2, photons=(1500, 400, 2600, 900)):
+    d, tex, _, src = scenes.travelling_wave_cube()
+    rng = np.random.default_rng(seed)
+    N = len(src["amp"])
+    amps = rng.uniform(0.0, 5e-9, (n_cfg, N)).astype(np.float32)
+    phases = rng.uniform(0.0, 2 * np.pi, (n_cfg, N)).astype(np.float32)
+    seeds = [scenes.splitmix64(seed * 1000 + i) for i in range(n_cfg)]
+    return d, tex, src, amps, phases, list(photons[:n_cfg]), seeds, (32, 32, 8)
+
+
+def test_pattern_batch_matches_single_scene_renders_and_adjoints(gpu_ctx):
+    d, tex, src, amps, phases, budgets, seeds, res = pattern_setup()
+    base = engine.Scene(gpu_ctx, d, tex, sources=src)
+    batch = engine.Batch.patterns(base, amps, phases, budgets, seeds, res)
+    imgs, st = batch.render()
+    imgs = imgs.double().cpu().numpy()
+    assert st["photons"] == sum(budgets)
+    X, Y, Z = res
+    yy, xx = np.mgrid[0:Y, 0:X]
+    mask = ((xx + 0.5 - X / 2) ** 2 + (yy + 0.5 - Y / 2) ** 2 <= 36).astype(np.float32)
+    G = torch.tensor(np.repeat(mask[None, None], len(budgets), axis=0).repeat(Z, axis=1), device="cuda")
+    grads, F, ast = batch.adjoint(G)
+    grads = grads.double().cpu().numpy()
+    assert ast["dropped"] == 0 and ast["photons"] == sum(budgets)
+    for i in range(len(budgets)):
+        sc = engine.Scene(gpu_ctx, d, tex, sources=dict(src, amp=amps[i], phase=phases[i]))
+        ref, _ = sc.render(budgets[i], res, seed=seeds[i])
+        ref = ref.double().cpu().numpy()
+        assert ref.sum() > 0
+        assert abs(imgs[i].sum() - ref.sum()) <= 1e-5 * ref.sum(), i
+        assert np.abs(imgs[i] - ref).max() <= 1e-4 * ref.max(), i
+        g, Fi, _ = sc.adjoint_render(G[i], budgets[i], seed=seeds[i])
+        g = g.double().cpu().numpy()
+        assert abs(F[i] - Fi) <= 1e-6 * abs(Fi), i
+        assert np.allclose(grads[i], g, rtol=2e-3, atol=1e-4 * np.abs(g).max()), i
