@@ -230,6 +230,59 @@ def adjoint_bench(args):
     return out
 
 
+def patterns_bench(args):
+    """Travelling-wave patterns: 16 point sources on a 10 mm ring (f 1 MHz,
+    a_i ~ U[0, 5e-9] m, phi_i ~ U[0, 2 pi)), sigma_s 2/mm, g 0.95, 64x64x32
+    sensor.  P patterns in one launch; each Adam step runs the batched
+    path-replay adjoint (objective and gradient of every pattern's 1 mm
+    aperture throughput w.r.t. its 16 amplitudes and 16 phases) and ascends."""
+    import numpy as np
+    import torch
+
+    from simpler_amd import engine, scenes
+
+    ctx = engine.Context(0)
+    d, tex, _, src = scenes.travelling_wave_cube()
+    res = (64, 64, 32)
+    base = engine.Scene(ctx, d, tex, sources=src)
+    P, n, N = args.patterns, args.pattern_photons, len(src["amp"])
+    rng = np.random.default_rng(7)
+    x = torch.tensor(np.concatenate([rng.uniform(0, 1, (P, N)), rng.uniform(0, 2 * np.pi, (P, N))], axis=1),
+                     dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.Adam([x], lr=0.05)
+    G = torch.tensor(scenes.aperture_weights(res), device="cuda").expand(P, -1, -1, -1).contiguous()
+    seeds = [scenes.splitmix64(11 * 1000003 + c) for c in range(P)]
+    hist, times, rev = [], [], 0
+    for step in range(args.adam_steps):
+        xa = x.detach().numpy()
+        b = engine.Batch.patterns(base, xa[:, :N] * 5e-9, xa[:, N:], [n] * P, seeds, res)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        grads, F, st = b.adjoint(G)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+        b.close()
+        g = grads.double().cpu()
+        opt.zero_grad()
+        x.grad = -torch.cat([g[:, :N] * 5e-9, g[:, N:2 * N]], dim=1)  # ascend the throughput
+        opt.step()
+        with torch.no_grad():
+            x[:, :N].clamp_(min=0.0)
+        hist.append(float(np.mean(F)))
+        rev += st["reverse_steps"]
+        eprint(f"patterns: Adam step {step + 1}/{args.adam_steps}: mean F {hist[-1]:.6g}, {times[-1]:.2f}s")
+    base.close()
+    ctx.close()
+    per_step = float(np.median(times))
+    rate = P * n / per_step
+    return dict(workload="16-element travelling-wave ring, f 1 MHz, sigma_s 2/mm, g 0.95, 64x64x32 sensor, "
+                         "1 mm aperture throughput",
+                patterns=P, paths_per_pattern=n, parameters_per_pattern=2 * N, adam_steps=args.adam_steps,
+                seconds_per_adam_step=per_step, adjoint_paths_per_s=rate, mean_objective=hist,
+                full_size_projection_s=(64 * 1e7 / rate) * 50,
+                full_size="64 patterns x 1e7 paths x 50 Adam steps at the measured rate")
+
+
 def cpu_baseline(args):
     from oracle import oracle as orc
     from simpler_amd import scenes
@@ -269,6 +322,29 @@ def copy_calibration(nbytes=1 << 30, reps=3):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def profile_workload(args):
+    """One launch of a named workload, for hardware-counter runs:
+    'spline' = the differentiable-spline forward render, 'adjoint' = its
+    path-replay adjoint (2^18 photons each)."""
+    import torch
+
+    from simpler_amd import engine, scenes
+
+    ctx = engine.Context(0)
+    d, tex, res, spl = scenes.spline_cube_ready(n=64)
+    sc = engine.Scene(ctx, d, tex, spline=spl)
+    n = 1 << 18
+    if args.profile == "spline":
+        _, st = sc.render(n, res, seed=11)
+    else:
+        G = torch.tensor(scenes.aperture_weights(res), device="cuda")
+        _, _, st = sc.adjoint_render(G, n, seed=11)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile": args.profile, "photons": n, "stats": st}), flush=True)
+    sc.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,6 +359,8 @@ def main():
     ap.add_argument("--cpu-photons-cutoff", type=int, default=768)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", choices=["spline", "adjoint"], default="",
+                    help="only run one launch of this workload (for hardware-counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling rocprof byte counters)")
     ap.add_argument("--no-batch", action="store_true")
@@ -291,12 +369,19 @@ def main():
                     help="photon budgets of the batched sweep times this factor (1.0: 5.5e8 photons)")
     ap.add_argument("--batch-replays", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-patterns", action="store_true")
+    ap.add_argument("--patterns", type=int, default=64)
+    ap.add_argument("--pattern-photons", type=int, default=4096)
+    ap.add_argument("--adam-steps", type=int, default=3)
     ap.add_argument("--adjoint-photons", type=int, default=1 << 19)
     ap.add_argument("--adjoint-fd", type=int, default=8, help="coefficients checked by central differences")
     ap.add_argument("--adjoint-eps", type=float, default=1e-4)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.profile:
+        profile_workload(args)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -341,6 +426,8 @@ def main():
         out["batched"] = batch_bench(args)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench(args)
+    if not args.no_patterns:
+        out["patterns"] = patterns_bench(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

@@ -143,6 +143,30 @@ __device__ __forceinline__ float draw_u(Lane &L) {
     return to_uniform(w);
 }
 
+// K consecutive uniforms of the current stream: the same values as K calls of
+// draw_u, from one Philox block, or two when the draws straddle a block
+// boundary (a photon's first draws and an NEE sub-stream's start at draw 0,
+// so their blocks are shared by the whole wave).  Two blocks cover any
+// K <= 5, and K <= 8 from a block boundary (the photon's first draws).
+template <int K>
+__device__ __forceinline__ void draw_k(Lane &L, float (&u)[K]) {
+    static_assert(K >= 1 && K <= 8, "at most two Philox blocks");
+    Cold &c = L.c;
+    const uint32_t d0 = c.draw, k0 = d0 & 3u;
+    uint32_t o[8];
+    philox4x32_10(c.pid_lo, c.pid_hi, d0 >> 2, c.stream, L.v.seed_lo, L.v.seed_hi, o);
+    if (k0 + K > 4u) philox4x32_10(c.pid_lo, c.pid_hi, (d0 >> 2) + 1u, c.stream, L.v.seed_lo, L.v.seed_hi, o + 4);
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        const uint32_t j = k0 + (uint32_t)i;  // 0..7
+        uint32_t w = o[0];
+#pragma unroll
+        for (int t = 1; t < (K <= 4 ? 4 + K : 8); ++t) w = j == (uint32_t)t ? o[t] : w;
+        u[i] = to_uniform(w);
+    }
+    c.draw = d0 + K;
+}
+
 template <int FIELD>
 __device__ __forceinline__ void refresh_field(Lane &L) {
     field_eval<FIELD>(L.S, L.v.f, L.h.p, L.h.scaling, L.h.tau, L.h.n0, L.h.g0);
@@ -301,10 +325,8 @@ __device__ __forceinline__ f3 bsdf_sample(Lane &L, f3 in, f3 nrm) {
     return draw_u(L) < F ? outR : outT;
 }
 
-// phase.cpp:39-76 and util.h:17-27
-__device__ __forceinline__ f3 hg_sample(Lane &L, f3 in) {
-    const float g = L.v.g;
-    const float sx = draw_u(L), sy = draw_u(L);
+// phase.cpp:39-76 and util.h:17-27, for the uniforms sx, sy
+__device__ __forceinline__ f3 hg_sample(float g, f3 in, float sx, float sy) {
     float cosT;
     if (fabsf(g) < kEps) {
         cosT = 1.f - 2.f * sx;
@@ -424,11 +446,13 @@ __device__ void scatter_once(Lane &L) {
         return;
     }
     Hot &h = L.h;
+    float u[3];  // HG (2), free path (1)
+    draw_k<3>(L, u);
     const float mag = len(h.v);
-    const f3 d1 = hg_sample(L, h.v / mag);
+    const f3 d1 = hg_sample(L.v.g, h.v / mag, u[0], u[1]);
     count(L, C_SCAT);
     h.v = mag * d1;
-    const float dist = -L.v.mfp * logf(draw_u(L));
+    const float dist = -L.v.mfp * logf(u[2]);
     flight_start(L, dist);
 }
 
@@ -474,7 +498,9 @@ __device__ void vertex(Lane &L) {
         c.stream = 1u + (uint32_t)c.depth;
         c.draw = 0;
         // scene.cpp:20-39 + warp.cpp:22-34: uniform hemisphere about -x
-        const float z = draw_u(L), s1 = draw_u(L);
+        float u[2];
+        draw_k<2>(L, u);
+        const float z = u[0], s1 = u[1];
         const float tmp = sqrtf(1.f - z * z);
         float sa, ca;
         sincosf(2.0f * kPi * s1, &sa, &ca);
@@ -678,14 +704,16 @@ __device__ void photon_init(Lane &L) {
     h.pl = mk3(0.f, 0.f, 0.f);
     h.Ll = 0.f;
     f3 pos = S.light_origin;
-    const int pixel = cdf_sample(S, draw_u(L));
+    float u[6];  // texel, position in texel (2), direction (2), duty phase (draws 0-5)
+    draw_k<6>(L, u);
+    const int pixel = cdf_sample(S, u[0]);
     const int py = pixel / S.tex_x;
     const int px = pixel - py * S.tex_x;
-    pos.y += -S.plane[0] / 2.0f + (float)px * S.pixsize[0] + draw_u(L) * S.pixsize[0];
-    pos.z += -S.plane[1] / 2.0f + (float)py * S.pixsize[1] + draw_u(L) * S.pixsize[1];
-    const float z = draw_u(L) * (1.f - S.ct) + S.ct;
+    pos.y += -S.plane[0] / 2.0f + (float)px * S.pixsize[0] + u[1] * S.pixsize[0];
+    pos.z += -S.plane[1] / 2.0f + (float)py * S.pixsize[1] + u[2] * S.pixsize[1];
+    const float z = u[3] * (1.f - S.ct) + S.ct;
     const float zt = sqrtf(1.f - z * z);
-    const float phi = draw_u(L) * 2.f * kPi;
+    const float phi = u[4] * 2.f * kPi;
     float sp, cp;
     sincosf(phi, &sp, &cp);
     f3 dir = mk3(-z, zt * cp, zt * sp);
@@ -705,7 +733,7 @@ __device__ void photon_init(Lane &L) {
         }
         return;
     }
-    h.tau = L.v.phi_min + L.v.phi_range * draw_u(L);
+    h.tau = L.v.phi_min + L.v.phi_range * u[5];
     h.scaling = fmaxf(fminf(sinf(h.tau), L.v.n_max_scaling), -L.v.n_max_scaling);
     h.p = pos;
     refresh_field<FIELD>(L);
