@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
     a.G.grid = A.grad;
     a.G.acc = sacc + threadIdx.x;
     a.G.stride = kBlock;
-    Lane L{h, cold[threadIdx.x], v, S, A, cnt, &a};
+    Lane L{h, cold[threadIdx.x], v, S, A, cnt, &a, nullptr};
     const int lane = __lane_id();
     while (true) {
         const unsigned long long need = __ballot(h.state == ST_NEED);

@@ -199,7 +199,14 @@ __device__ __forceinline__ void field_eval<kSources>(const DevScene &S, const Fi
     g = gacc;
 }
 
+// Mirror-wrapped node index (spline.h:617-620, 636-644 for 1-D / 2-D; also
+// applied in 3-D).  One reflection covers i in [-(n-1), 2n-2], i.e. every
+// point within the grid's box and up to a grid length beyond it, without the
+// integer division of the general period formula, which is kept for the rest.
 __device__ __forceinline__ int mirror_idx(int i, int n) {
+    int r = i < 0 ? -i : i;
+    r = r >= n ? 2 * n - 2 - r : r;
+    if ((unsigned)r < (unsigned)n) return r;
     const int period = 2 * n - 2;
     int w = i % period;
     w += (w < 0) ? period : 0;

@@ -129,6 +129,7 @@ struct Lane {
     const RenderArgs &A;
     uint32_t *cnt;          // block-level event counters in LDS
     struct AdjState *a;     // adjoint state (adjoint kernels only)
+    float4 *mc;             // n, grad n at the saved main-path point (LDS; render kernels)
 };
 
 __device__ __forceinline__ void count(Lane &L, int k) { atomicAdd(L.cnt + k, 1u); }
@@ -170,6 +171,20 @@ __device__ __forceinline__ void draw_k(Lane &L, float (&u)[K]) {
 template <int FIELD>
 __device__ __forceinline__ void refresh_field(Lane &L) {
     field_eval<FIELD>(L.S, L.v.f, L.h.p, L.h.scaling, L.h.tau, L.h.n0, L.h.g0);
+}
+
+// n and grad n at the main-path point just restored: the saved values where
+// the kernel keeps them (they are the evaluation at that very point), else a
+// fresh evaluation.
+template <int FIELD>
+__device__ __forceinline__ void restore_field(Lane &L) {
+    if (L.mc) {
+        const float4 m = *L.mc;
+        L.h.n0 = m.x;
+        L.h.g0 = mk3(m.y, m.z, m.w);
+    } else {
+        refresh_field<FIELD>(L);
+    }
 }
 
 __device__ __forceinline__ void start_segment(Lane &L, float dist) {
@@ -494,6 +509,7 @@ __device__ void vertex(Lane &L) {
         c.mp = h.p;
         c.mv = h.v;
         c.mL = h.L;
+        if (L.mc) *L.mc = make_float4(h.n0, h.g0.x, h.g0.y, h.g0.z);
         c.main_draw = c.draw;
         c.stream = 1u + (uint32_t)c.depth;
         c.draw = 0;
@@ -526,14 +542,14 @@ __device__ void direct_continue(Lane &L, int res) {
         return;
     }
     restore_main(L, S.direct_leak ? L.h.L : L.c.mL);
-    refresh_field<FIELD>(L);
+    restore_field<FIELD>(L);
     scatter_start(L);
 }
 
 template <int FIELD>
 __device__ void nee_continue(Lane &L) {
     restore_main(L, L.c.mL);
-    refresh_field<FIELD>(L);
+    restore_field<FIELD>(L);
     scatter_once(L);
 }
 
@@ -741,6 +757,7 @@ __device__ void photon_init(Lane &L) {
     h.L = total;
     if (S.use_direct || per_ray(MODE)) {
         if (L.a) adj_sub_begin(*L.a);
+        if (L.mc) *L.mc = make_float4(h.n0, h.g0.x, h.g0.y, h.g0.z);
         c.mp = h.p;
         c.mv = h.v;
         c.mL = h.L;
@@ -770,14 +787,18 @@ namespace simpler {
 // intervals): the J_m / J_m' table is staged in LDS once per workgroup, so the
 // per-step lookup is an LDS read instead of an L1/L2 round trip.
 //
-// Occupancy: at least 5 waves per SIMD (<= 96 VGPRs; the few spills fall in
-// the event code); measured 3-6 % faster than the unconstrained 4 waves.
+// Occupancy: the Bessel kernel runs at least 5 waves per SIMD (<= 96 VGPRs;
+// the few spills fall in the event code), measured 3-6 % faster than 4 waves;
+// the spline and point-source kernels keep the registers their evaluation
+// needs (forcing 96 VGPRs spills inside their march step).
 template <int FIELD, int MODE, bool BATCH, bool JLDS = false>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_render(DevScene S, RenderArgs A) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FIELD == kBessel ? 5 : (FIELD == kSpline ? 3 : 1))))
+k_render(DevScene S, RenderArgs A) {
     __shared__ Cold cold[kBlock];
     __shared__ uint32_t cnt[C_N];
     __shared__ Var vars[BATCH ? kBlock : 1];
     __shared__ float4 jlds[JLDS ? 4 * kJtabLds : 1];
+    __shared__ float4 mcache[kBlock];
     if (threadIdx.x < C_N) cnt[threadIdx.x] = 0u;
     if (JLDS)
         for (int i = threadIdx.x; i < 4 * S.jtab_n; i += blockDim.x) jlds[i] = S.jtab[i];
@@ -789,7 +810,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     h.steps = 0;
     h.state = ST_NEED;
     h.dead = false;
-    Lane L{h, cold[threadIdx.x], v, S, A, cnt, nullptr};
+    Lane L{h, cold[threadIdx.x], v, S, A, cnt, nullptr, &mcache[threadIdx.x]};
     const int lane = __lane_id();
     while (true) {
         // fetch: one atomic per wave for all lanes that need a photon
