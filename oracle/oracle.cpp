@@ -685,8 +685,8 @@ struct Tracer {
     }
 
     /* camera.cpp:53-75 with lens.cpp:69-97 */
-    bool lens_deflect(const float *origin, float aperture, float focal, V3<R> &pos, V3<R> &dir,
-                      R &total) const {
+    static bool lens_deflect(const float *origin, float aperture, float focal, V3<R> &pos, V3<R> &dir,
+                             R &total) {
         (void)origin;
         R sq = pos.y * pos.y + pos.z * pos.z;
         if (sq > (R)aperture * (R)aperture) return false;
@@ -833,6 +833,9 @@ struct Tracer {
     /* scene.cpp:20-39 with warp.cpp:22-34 */
     V3<R> sampleRandomDirection() {
         const R s0 = rnd(), s1 = rnd();
+        return hemisphere_dir(s0, s1);
+    }
+    static V3<R> hemisphere_dir(R s0, R s1) {
         const R z = s0;
         const R tmp = std::sqrt(R(1) - z * z);
         const R ang = R(2) * K<R>::PI * s1;
@@ -901,10 +904,9 @@ struct Tracer {
     }
 
     /* pmf.h:109-121 */
-    int cdf_sample(R u) const {
-        const float *c = S.cdf.data();
-        const long n = (long)S.cdf.size();
-        const long pos = std::lower_bound(c, c + n, (float)u) - c;
+    int cdf_sample(R u) const { return cdf_search(S.cdf.data(), (long)S.cdf.size(), (float)u); }
+    static int cdf_search(const float *c, long n, float u) {
+        const long pos = std::lower_bound(c, c + n, u) - c;
         long index = std::min(n - 2, std::max(0L, pos - 1));
         while (c[index + 1] - c[index] == 0 && index < n - 2) ++index;
         return (int)index;
@@ -1227,6 +1229,33 @@ int oracle_render(void *hv, int precision, int64_t n_photons, int64_t n_total, u
     Scene<float> S;
     fill_scene(S, h, n_total);
     return render(S, n_photons, seed, offset, cutoff, image, X, Y, Z, st, threads);
+}
+
+/* Single path functions in FP64, for the distribution / closed-form pins. */
+int oracle_lens_deflect(double aperture, double focal, double pos[3], double dir[3], double *total) {
+    V3<double> p(pos[0], pos[1], pos[2]), d(dir[0], dir[1], dir[2]);
+    double t = *total;
+    const bool ok = Tracer<double>::lens_deflect(nullptr, (float)aperture, (float)focal, p, d, t);
+    for (int i = 0; i < 3; ++i) {
+        pos[i] = p[i];
+        dir[i] = d[i];
+    }
+    *total = t;
+    return ok ? 1 : 0;
+}
+
+void oracle_hemisphere(int n, const double *u0, const double *u1, double *out) {
+    for (int i = 0; i < n; ++i) {
+        const V3<double> d = Tracer<double>::hemisphere_dir(u0[i], u1[i]);
+        for (int k = 0; k < 3; ++k) out[3 * i + k] = d[k];
+    }
+}
+
+int oracle_cdf_sample(void *hv, int n, const float *u, int *out) {
+    const OracleSceneHandle &h = *(OracleSceneHandle *)hv;
+    const long m = (long)h.cdf.size();
+    for (int i = 0; i < n; ++i) out[i] = Tracer<double>::cdf_search(h.cdf.data(), m, u[i]);
+    return 0;
 }
 
 double oracle_dead_path_cut(void *hv) {

@@ -65,6 +65,11 @@ def lib(fast=False):
     L.oracle_spline_eval2d.argtypes = [dp, ip, dp, dp, C.c_int, C.c_int, C.c_int, dp, dp]
     L.oracle_spline_build3d.argtypes = [dp, ip, dp]
     L.oracle_spline_eval3d.argtypes = [dp, ip, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
+    L.oracle_lens_deflect.argtypes = [C.c_double, C.c_double, dp, dp, C.POINTER(C.c_double)]
+    L.oracle_lens_deflect.restype = C.c_int
+    L.oracle_hemisphere.argtypes = [C.c_int, dp, dp, dp]
+    L.oracle_cdf_sample.argtypes = [P, C.c_int, fp, ip]
+    L.oracle_cdf_sample.restype = C.c_int
     _LIBS[name] = L
     return L
 
@@ -130,6 +135,13 @@ class OracleScene:
                              C.byref(st), threads)
         return img.reshape(Z, Y, X), st.as_dict()
 
+    def cdf_sample(self, u):
+        """Texel indices that the source's texture CDF maps the uniforms u to."""
+        u = np.ascontiguousarray(u, np.float32)
+        out = np.zeros(len(u), np.int32)
+        self.L.oracle_cdf_sample(self.h, len(u), u, out)
+        return out
+
     def dead_path_cut(self):
         return self.L.oracle_dead_path_cut(self.h)
 
@@ -159,6 +171,27 @@ class OracleScene:
             self.L.oracle_scene_destroy(self.h)
         except Exception:
             pass
+
+
+# single path functions (double precision) --------------------------------
+def lens_deflect(aperture, focal, pos, d, total=0.0):
+    """Thin-lens deflection of a ray at the lens plane x = const (optical axis
+    -x).  Returns (passed, pos, dir, total) with the total path length
+    corrected for the lens."""
+    p = np.array(pos, np.float64)
+    v = np.array(d, np.float64)
+    t = C.c_double(total)
+    ok = lib().oracle_lens_deflect(aperture, focal, p, v, C.byref(t))
+    return bool(ok), p, v, t.value
+
+
+def hemisphere(u0, u1):
+    """Next-event directions for the uniform pairs (u0, u1), shape (n, 3)."""
+    u0 = np.ascontiguousarray(u0, np.float64)
+    u1 = np.ascontiguousarray(u1, np.float64)
+    out = np.zeros((len(u0), 3))
+    lib().oracle_hemisphere(len(u0), u0, u1, out.reshape(-1))
+    return out
 
 
 # spline helpers (double precision, for the reference CSV tests) ----------

@@ -105,8 +105,7 @@ def _bin_check(g, o):
     return int(bad.sum()), abs(Tg.sum() - To.sum()), tot_sig, int(((Tg > 0) | (To > 0)).sum())
 
 
-def test_render_every_bin_within_3_sigma_of_oracle(gpu_ctx):
-    d, tex, res = scenes.bessel_cube_small()
+def _image_parity(gpu_ctx, d, tex, res, seed=3, what="image"):
     gs = engine.Scene(gpu_ctx, d, tex)
     osc = O.OracleScene(d, tex)
     B, n = 8, 48
@@ -114,25 +113,47 @@ def test_render_every_bin_within_3_sigma_of_oracle(gpu_ctx):
     g_st, o_st = [], []
 
     def rg(b):
-        img, st = gs.render(n, res, seed=3, offset=b * n, n_total=N)
+        img, st = gs.render(n, res, seed=seed, offset=b * n, n_total=N)
         g_st.append(st)
         return img.double().cpu().numpy()
 
     def ro(b):
-        img, st = osc.render(n, res, seed=3, offset=b * n, n_total=N, precision=32, threads=gpu_threads(),
+        img, st = osc.render(n, res, seed=seed, offset=b * n, n_total=N, precision=32, threads=gpu_threads(),
                              cutoff=True)
         o_st.append(st)
         return img
 
     g, o = _batches(rg, B), _batches(ro, B)
     bad, dtot, tot_sig, nbins = _bin_check(g, o)
-    log(f"bins {nbins} bad {bad} total diff {dtot:.4g} sigma {tot_sig:.4g}")
+    log(f"{what}: bins {nbins} bad {bad} total diff {dtot:.4g} sigma {tot_sig:.4g}")
+    assert nbins > 0
     assert bad == 0
     assert dtot <= 3 * tot_sig
     for k in ("photons", "scatters", "direct_splats"):
         assert sum(s[k] for s in g_st) == sum(s[k] for s in o_st), k
     ng, no = sum(s["nee_splats"] for s in g_st), sum(s["nee_splats"] for s in o_st)
     assert abs(ng - no) <= 0.002 * no + 2
+
+
+def test_render_every_bin_within_3_sigma_of_oracle(gpu_ctx):
+    d, tex, res = scenes.bessel_cube_small()
+    _image_parity(gpu_ctx, d, tex, res)
+
+
+@pytest.mark.parametrize("lenses", ["emitter", "sensor", "emitter_cone"])
+def test_lensed_render_within_3_sigma_of_oracle(gpu_ctx, lenses):
+    """Thin lenses (lens.cpp:19-33,69-97): the emitter lens focuses the
+    collimated beam 15 mm behind the source face, or a 10-degree cone; the
+    sensor lens refocuses the exit rays (and stops those outside its 3 mm
+    aperture)."""
+    d, tex, res = scenes.bessel_cube_small()
+    if lenses.startswith("emitter"):
+        d.set(emitter_lens_active=1, emitter_lens_focal_length=0.015, emitter_lens_aperture=0.0015)
+    if lenses == "emitter_cone":
+        d.set(half_theta_limit=math.radians(10.0))
+    if lenses == "sensor":
+        d.set(sensor_lens_active=1, sensor_lens_focal_length=0.02, sensor_lens_aperture=0.003)
+    _image_parity(gpu_ctx, d, tex, res, seed=17, what=lenses)
 
 
 def test_render_statistics_with_independent_seeds(gpu_ctx):
