@@ -288,8 +288,11 @@ public:
     Float getMaxPathlength() const { return m_maxPathlength; }
     bool getUseDirect() const { return m_useDirect; }
 
-private:
+    // the engine descriptor a render of (medium, scene) uploads: the scene's
+    // descriptor with the medium's and this renderer's settings filled in
     simpler_scene_desc full_desc(const med::Medium &medium, const scn::Scene<vector_type> &scene) const;
+
+private:
     int m_maxDepth;
     Float m_maxPathlength;
     bool m_useDirect, m_useAngularSampling;
