@@ -35,6 +35,11 @@ FP32_PEAK_TFLOPS = 157.3
 # scaled by the counters' reading for a 1 GiB device copy (x2.00 fetch,
 # x1.00 write): almost all of it is the splat atomics.
 TRAFFIC_BYTES_PER_PHOTON = 1284.85
+HBM_PEAK_GBS = 8000.0
+# HBM bytes per sample of the three prefilter passes on a 512^3 grid
+# (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of one call, same scaling):
+# y 8.07, x 14.43, z 8.03
+PREFILTER_TRAFFIC_BYTES_PER_SAMPLE = 30.54
 
 
 def eprint(*a):
@@ -283,6 +288,53 @@ def patterns_bench(args):
                 full_size="64 patterns x 1e7 paths x 50 Adam steps at the measured rate")
 
 
+def prefilter_bench(args):
+    """GPU B-spline prefilter of an n^3 grid of node values (in place, three
+    passes y, x, z): HIP-event time per call over the timed calls, and the
+    rate against 24 B per sample (each pass reads and writes every sample
+    once).  CPU: the oracle's FP64 restatement of the reference's build3d on
+    a 256^3 grid, one core."""
+    import numpy as np
+    import torch
+
+    from oracle import oracle as orc
+    from simpler_amd import engine
+
+    ctx = engine.Context(0)
+    n = args.prefilter_n
+    dims = (n, n, n)
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    src = 1.3333 + 1e-4 * torch.randn(n ** 3, device="cuda", generator=gen)
+    buf = src.clone()
+    ctx.spline_prefilter(buf, dims)  # warmup
+    times = []
+    for _ in range(args.prefilter_reps):
+        buf.copy_(src)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ctx.spline_prefilter(buf, dims)
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ms = sum(times) / len(times)
+    algo_bytes = 24 * n ** 3
+    gbs = algo_bytes / (ms / 1e3) / 1e9
+    m = 256
+    grid = 1.3333 + 1e-4 * np.random.default_rng(0).standard_normal(m ** 3)
+    t = time.perf_counter()
+    orc.spline_build3d(grid, [m, m, m])
+    cpu_s = time.perf_counter() - t
+    eprint(f"prefilter {n}^3: {ms:.3f} ms per call, {gbs:.0f} GB/s algorithmic; CPU {m}^3 in {cpu_s:.2f} s")
+    ctx.close()
+    return dict(grid=list(dims), ms_per_call=ms, calls=len(times), algorithmic_bytes_per_sample=24,
+                roofline={"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": gbs / HBM_PEAK_GBS, "traffic": PREFILTER_TRAFFIC_BYTES_PER_SAMPLE * n ** 3
+                          if PREFILTER_TRAFFIC_BYTES_PER_SAMPLE else None},
+                samples_per_s=n ** 3 / (ms / 1e3),
+                cpu_baseline=dict(value=m ** 3 / cpu_s, unit="samples/s", cores=1, kind="port",
+                                  sample=f"oracle build3d (FP64) of a {m}^3 grid, {cpu_s:.2f} s"))
+
+
 def cpu_baseline(args):
     from oracle import oracle as orc
     from simpler_amd import scenes
@@ -308,6 +360,20 @@ def cpu_baseline(args):
                 same_algorithm_as_gpu=dict(value=nc / dtc, photons=nc, seconds=dtc,
                                            er_steps_per_s=stc["er_steps"] / dtc,
                                            note="same CPU code with the dead-path cutoff the GPU uses"))
+
+
+def profile_prefilter(args):
+    """One in-place prefilter of an n^3 grid (for hardware-counter runs)."""
+    import torch
+
+    from simpler_amd import engine
+    ctx = engine.Context(0)
+    n = args.prefilter_n
+    buf = 1.3333 + 1e-4 * torch.randn(n ** 3, device="cuda")
+    ctx.spline_prefilter(buf, (n, n, n))
+    torch.cuda.synchronize()
+    print(json.dumps({"profile": "prefilter", "grid": [n, n, n]}), flush=True)
+    ctx.close()
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):
@@ -359,7 +425,10 @@ def main():
     ap.add_argument("--cpu-photons-cutoff", type=int, default=768)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile", choices=["spline", "adjoint"], default="",
+    ap.add_argument("--no-prefilter", action="store_true")
+    ap.add_argument("--prefilter-n", type=int, default=512)
+    ap.add_argument("--prefilter-reps", type=int, default=5)
+    ap.add_argument("--profile", choices=["spline", "adjoint", "prefilter"], default="",
                     help="only run one launch of this workload (for hardware-counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling rocprof byte counters)")
@@ -379,6 +448,9 @@ def main():
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.profile == "prefilter":
+        profile_prefilter(args)
         return
     if args.profile:
         profile_workload(args)
@@ -428,6 +500,8 @@ def main():
         out["adjoint"] = adjoint_bench(args)
     if not args.no_patterns:
         out["patterns"] = patterns_bench(args)
+    if not args.no_prefilter:
+        out["prefilter"] = prefilter_bench(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

@@ -200,6 +200,25 @@ int simpler_scene_set_spline(simpler_scene *scene, const float *coeffs_host,
                              const int32_t dims[3], const float xmin[3],
                              const float xmax[3]);
 
+/* Tricubic B-spline field from grid SAMPLES (node values, x fastest), with
+ * the interpolation prefilter run on the GPU (replaces the host build3d of
+ * spline.h:686-764 that rif_interpolated's VOL constructor runs,
+ * spline.h:115-162 / rif_interpolated.h:101-104).  samples may be host or
+ * device memory (samples_on_device); fails with SIMPLER_ERR_INVALID on a
+ * non-finite sample.  Synchronous. */
+int simpler_scene_set_spline_samples(simpler_scene *scene, const float *samples,
+                                     int samples_on_device, const int32_t dims[3],
+                                     const float xmin[3], const float xmax[3]);
+
+/* The scene's B-spline coefficients (absolute values, x fastest) into host
+ * memory; SIMPLER_ERR_STATE if the scene has none.  Synchronous. */
+int simpler_scene_get_spline(const simpler_scene *scene, float *coeffs_host);
+
+/* In-place B-spline interpolation prefilter of a device grid (spline.h:
+ * 686-764: mirror-boundary recursive filter, pole sqrt(3)-2, gain 6, along
+ * y, then x, then z) on the context's stream; dims in [2, 4096]. */
+int simpler_spline_prefilter(simpler_ctx *ctx, float *grid_dev, const int32_t dims[3]);
+
 /* ---- the hot path ---------------------------------------------------- */
 /*
  * Render photons [photon_offset, photon_offset + n_photons) of a render of

@@ -29,7 +29,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["render.hip"]
-HIP_DEPS = ["render.hip", "fields.h", "device_math.h", "adjoint.h", "adjoint_kernel.h"]
+HIP_DEPS = ["render.hip", "fields.h", "device_math.h", "adjoint.h", "adjoint_kernel.h", "prefilter.h"]
 
 
 def _run(cmd):

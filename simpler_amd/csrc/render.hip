@@ -776,6 +776,7 @@ __device__ void photon_init(Lane &L) {
 
 }  // namespace simpler
 #include "adjoint_kernel.h"
+#include "prefilter.h"
 namespace simpler {
 
 // The persistent kernel.  BATCH: photons of many configurations (shared
@@ -909,6 +910,7 @@ struct simpler_ctx {
     float *d_log = nullptr;  // adjoint segment logs (grown on demand)
     size_t log_bytes = 0;
     unsigned long long *d_adj = nullptr;  // adjoint objective (double) + counters
+    unsigned *d_pref = nullptr;           // prefilter: non-finite flag, minimum key
     std::string err;
 };
 
@@ -1284,6 +1286,7 @@ int simpler_ctx_destroy(simpler_ctx *ctx) {
     (void)hipFree(ctx->d_scratch);
     (void)hipFree(ctx->d_log);
     (void)hipFree(ctx->d_adj);
+    (void)hipFree(ctx->d_pref);
     (void)hipStreamDestroy(ctx->own);
     delete ctx;
     return SIMPLER_OK;
@@ -1519,15 +1522,14 @@ int simpler_scene_set_sources(simpler_scene *sc, int n_src, const float *pos, co
     return SIMPLER_OK;
 }
 
+static int check_grid(simpler_ctx *ctx, const int32_t dims[3], const float *xmin, const float *xmax);
+
 int simpler_scene_set_spline(simpler_scene *sc, const float *coeffs, const int32_t dims[3], const float xmin[3],
                              const float xmax[3]) {
     if (!sc) return SIMPLER_ERR_INVALID;
     simpler_ctx *ctx = sc->ctx;
     if (!coeffs || !dims || !xmin || !xmax) return fail(ctx, SIMPLER_ERR_INVALID, "spline: null argument");
-    for (int i = 0; i < 3; ++i) {
-        if (dims[i] < 2 || dims[i] > 4096) return fail(ctx, SIMPLER_ERR_INVALID, "spline: dims must be in [2,4096]");
-        if (!(xmax[i] > xmin[i])) return fail(ctx, SIMPLER_ERR_INVALID, "spline: xmax must exceed xmin");
-    }
+    if (int rc = check_grid(ctx, dims, xmin, xmax)) return rc;
     const size_t n = (size_t)dims[0] * dims[1] * dims[2];
     // store c - ref with ref the coefficient mean (the device adds ref back)
     double mean = 0.0;
@@ -1553,6 +1555,104 @@ int simpler_scene_set_spline(simpler_scene *sc, const float *coeffs, const int32
         sc->sxmin[i] = xmin[i];
         sc->sxmax[i] = xmax[i];
     }
+    return SIMPLER_OK;
+}
+
+static int check_grid(simpler_ctx *ctx, const int32_t dims[3], const float *xmin, const float *xmax) {
+    for (int i = 0; i < 3; ++i) {
+        if (dims[i] < 2 || dims[i] > 4096) return fail(ctx, SIMPLER_ERR_INVALID, "spline: dims must be in [2,4096]");
+        if (xmin && !(xmax[i] > xmin[i])) return fail(ctx, SIMPLER_ERR_INVALID, "spline: xmax must exceed xmin");
+    }
+    return SIMPLER_OK;
+}
+
+static int prefilter_buffer(simpler_ctx *ctx) {
+    if (!ctx->d_pref) HIP_TRY(ctx, hipMalloc(&ctx->d_pref, 2 * sizeof(unsigned)));
+    return SIMPLER_OK;
+}
+
+int simpler_spline_prefilter(simpler_ctx *ctx, float *grid, const int32_t dims[3]) {
+    if (!ctx) return SIMPLER_ERR_INVALID;
+    if (!grid || !dims) return fail(ctx, SIMPLER_ERR_INVALID, "prefilter: null argument");
+    if (int rc = check_grid(ctx, dims, nullptr, nullptr)) return rc;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (int rc = prefilter_buffer(ctx)) return rc;
+    const int N[3] = {dims[0], dims[1], dims[2]};
+    HIP_TRY(ctx, launch_prefilter3d(grid, N, 0.f, ctx->d_pref, ctx->cur));
+    return SIMPLER_OK;
+}
+
+int simpler_scene_set_spline_samples(simpler_scene *sc, const float *samples, int samples_on_device,
+                                     const int32_t dims[3], const float xmin[3], const float xmax[3]) {
+    if (!sc) return SIMPLER_ERR_INVALID;
+    simpler_ctx *ctx = sc->ctx;
+    if (!samples || !dims || !xmin || !xmax) return fail(ctx, SIMPLER_ERR_INVALID, "spline: null argument");
+    if (int rc = check_grid(ctx, dims, xmin, xmax)) return rc;
+    const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
+    if (int rc = prefilter_buffer(ctx)) return rc;
+    (void)hipFree(sc->d_spline);
+    sc->d_spline = nullptr;
+    float *d = nullptr;
+    HIP_TRY(ctx, hipMalloc(&d, n * sizeof(float)));
+    // the shift (stored as spline_ref, added back by the field evaluation) is
+    // the first sample: any value inside the data's range keeps FP32
+    // resolution on the variation
+    float shift = 0.f;
+    unsigned w[2] = {0u, 0u};
+    hipError_t e = hipMemcpyAsync(d, samples, n * sizeof(float),
+                                  samples_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->cur);
+    if (e == hipSuccess) e = hipMemcpyAsync(&shift, d, sizeof(float), hipMemcpyDeviceToHost, ctx->cur);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->cur);
+    if (e == hipSuccess && !std::isfinite(shift)) {
+        (void)hipFree(d);
+        return fail(ctx, SIMPLER_ERR_INVALID, "spline: non-finite sample");
+    }
+    const int N[3] = {dims[0], dims[1], dims[2]};
+    if (e == hipSuccess) e = launch_prefilter3d(d, N, shift, ctx->d_pref, ctx->cur);
+    if (e == hipSuccess) e = hipMemcpyAsync(w, ctx->d_pref, sizeof(w), hipMemcpyDeviceToHost, ctx->cur);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->cur);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return fail(ctx, SIMPLER_ERR_HIP, std::string("spline prefilter: ") + hipGetErrorString(e));
+    }
+    if (w[0]) {
+        (void)hipFree(d);
+        return fail(ctx, SIMPLER_ERR_INVALID, "spline: non-finite sample");
+    }
+    sc->d_spline = d;
+    sc->spline_ref = shift;
+    sc->spline_min = key_to_float(w[1]) + shift;
+    for (int i = 0; i < 3; ++i) {
+        sc->sN[i] = dims[i];
+        sc->sxmin[i] = xmin[i];
+        sc->sxmax[i] = xmax[i];
+    }
+    return SIMPLER_OK;
+}
+
+__global__ void k_add_f32(const float *a, float *out, size_t n, float v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] + v;
+}
+
+int simpler_scene_get_spline(const simpler_scene *sc, float *coeffs_host) {
+    if (!sc) return SIMPLER_ERR_INVALID;
+    simpler_ctx *ctx = sc->ctx;
+    if (!coeffs_host) return fail(ctx, SIMPLER_ERR_INVALID, "spline: null argument");
+    if (!sc->d_spline) return fail(ctx, SIMPLER_ERR_STATE, "spline: the scene has no coefficients");
+    const size_t n = (size_t)sc->sN[0] * sc->sN[1] * sc->sN[2];
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    float *tmp = nullptr;
+    HIP_TRY(ctx, hipMalloc(&tmp, n * sizeof(float)));
+    hipLaunchKernelGGL(k_add_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->cur, sc->d_spline, tmp, n,
+                       sc->spline_ref);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(coeffs_host, tmp, n * sizeof(float), hipMemcpyDeviceToHost, ctx->cur);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->cur);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(ctx, SIMPLER_ERR_HIP, std::string("spline readback: ") + hipGetErrorString(e));
     return SIMPLER_OK;
 }
 

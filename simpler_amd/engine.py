@@ -54,6 +54,9 @@ def load_library():
     L.simpler_scene_get_desc.argtypes = [P, C.POINTER(SceneDesc)]
     L.simpler_scene_set_sources.argtypes = [P, C.c_int, fptr, fptr, fptr, C.c_int, fptr]
     L.simpler_scene_set_spline.argtypes = [P, fptr, C.POINTER(C.c_int32), fptr, fptr]
+    L.simpler_scene_set_spline_samples.argtypes = [P, P, C.c_int, C.POINTER(C.c_int32), fptr, fptr]
+    L.simpler_scene_get_spline.argtypes = [P, fptr]
+    L.simpler_spline_prefilter.argtypes = [P, P, C.POINTER(C.c_int32)]
     L.simpler_render.argtypes = [P, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, P, C.c_int, C.c_int, C.c_int,
                                  C.c_int, C.POINTER(Stats)]
     L.simpler_render_host.argtypes = [P, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, fptr, C.c_int, C.c_int,
@@ -83,7 +86,8 @@ def load_library():
                  "simpler_field_eval", "simpler_read_stats", "simpler_batch_create", "simpler_batch_render",
                  "simpler_batch_graph_capture", "simpler_batch_graph_launch", "simpler_batch_destroy",
                  "simpler_adjoint_param_count", "simpler_adjoint_render", "simpler_adjoint_rays",
-                 "simpler_autotune", "simpler_batch_create_sources", "simpler_batch_adjoint"):
+                 "simpler_autotune", "simpler_batch_create_sources", "simpler_batch_adjoint",
+                 "simpler_scene_set_spline_samples", "simpler_scene_get_spline", "simpler_spline_prefilter"):
         getattr(L, name).restype = C.c_int
     _LIB = L
     return L
@@ -118,6 +122,14 @@ class Context:
         if rc != 0:
             msg = self.L.simpler_last_error(self.h).decode()
             raise SimplerError(f"simpler call failed ({rc}): {msg}")
+
+    def spline_prefilter(self, grid, dims):
+        """In-place B-spline prefilter of a contiguous float32 CUDA tensor of
+        node values (x fastest) on this context's stream."""
+        if grid.dtype.itemsize != 4 or not grid.is_contiguous() or not grid.is_cuda or grid.numel() != int(
+                np.prod(dims)):
+            raise SimplerError("grid must be a contiguous float32 CUDA tensor matching dims")
+        self.check(self.L.simpler_spline_prefilter(self.h, C.c_void_p(grid.data_ptr()), (C.c_int32 * 3)(*dims)))
 
     def set_stream(self, stream_ptr):
         self.check(self.L.simpler_ctx_set_stream(self.h, C.c_void_p(stream_ptr)))
@@ -193,6 +205,35 @@ class Scene:
         self.ctx.check(self.L.simpler_scene_set_spline(self.h, _fp(c), (C.c_int32 * 3)(*dims),
                                                        (C.c_float * 3)(*xmin), (C.c_float * 3)(*xmax)))
         self._spline = (tuple(dims), tuple(xmin), tuple(xmax), c)
+
+    def set_spline_samples(self, samples, dims, xmin, xmax):
+        """Spline field from grid node values (x fastest): a numpy array or a
+        float32 CUDA tensor; the B-spline prefilter runs on the GPU."""
+        import torch
+        if isinstance(samples, torch.Tensor):
+            if samples.dtype != torch.float32 or not samples.is_contiguous() or not samples.is_cuda:
+                raise SimplerError("samples must be a contiguous float32 CUDA tensor")
+            ptr, on_dev, keep = samples.data_ptr(), 1, samples
+        else:
+            keep = np.ascontiguousarray(samples, np.float32).reshape(-1)
+            ptr, on_dev = keep.ctypes.data, 0
+        count = keep.numel() if on_dev else keep.size
+        if count != int(np.prod(dims)):
+            raise SimplerError("samples do not match dims")
+        self.ctx.check(self.L.simpler_scene_set_spline_samples(self.h, C.c_void_p(ptr), on_dev,
+                                                               (C.c_int32 * 3)(*dims), (C.c_float * 3)(*xmin),
+                                                               (C.c_float * 3)(*xmax)))
+        self._spline = (tuple(dims), tuple(xmin), tuple(xmax), None)
+        self._spline = (tuple(dims), tuple(xmin), tuple(xmax), self.spline_coeffs())
+
+    def spline_coeffs(self):
+        """The scene's B-spline coefficients (float32, x fastest)."""
+        dims = self._spline[0] if hasattr(self, "_spline") else None
+        if dims is None:
+            raise SimplerError("the scene has no spline field")
+        out = np.zeros(int(np.prod(dims)), np.float32)
+        self.ctx.check(self.L.simpler_scene_get_spline(self.h, _fp(out)))
+        return out
 
     def params(self):
         """The differentiable field parameters as a float64 vector: spline
