@@ -38,8 +38,8 @@ TRAFFIC_BYTES_PER_PHOTON = 1284.85
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per sample of the three prefilter passes on a 512^3 grid
 # (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of one call, same scaling):
-# y 8.07, x 14.43, z 8.03
-PREFILTER_TRAFFIC_BYTES_PER_SAMPLE = 30.54
+# y 8.07, x 8.00, z 8.03
+PREFILTER_TRAFFIC_BYTES_PER_SAMPLE = 24.10
 
 
 def eprint(*a):

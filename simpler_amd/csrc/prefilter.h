@@ -175,6 +175,141 @@ __global__ void __launch_bounds__(kPrefBlock)
     }
 }
 
+// The x pass with LDS staging: a one-wave workgroup owns 64 rows (one per
+// lane) and a 64 x B tile of LDS.  Loads and stores move through the tile so
+// that every global instruction covers 8 rows x 128 contiguous bytes (whole
+// cache lines); each lane runs its row's recursions on the tile's row, with
+// the same windows as k_prefilter.  With one wave per workgroup the barriers
+// order the tile's writes and reads without making waves wait on each other.
+constexpr int kPrefRowBlock = 64;
+
+template <int B, int D>
+__global__ void __launch_bounds__(kPrefRowBlock)
+    k_prefilter_rows(float *__restrict__ c, int n, long rows) {
+    constexpr int W = B + D;
+    constexpr int H = (W + B - 1) / B;  // tiles per window
+    constexpr int Q = B / 4;            // float4 per row and tile
+    static_assert(B % 4 == 0 && D <= B, "window layout");
+    __shared__ float t[64][B + 1];
+    const int lane = threadIdx.x;
+    const long row0 = (long)blockIdx.x * 64;
+    const bool vec = (n % 4) == 0;  // every row starts 16-byte aligned
+    // tile <- columns [c0, c0 + B) of the wave's rows (columns past n clamp to n - 1)
+    auto stage_in = [&](int c0) {
+#pragma unroll
+        for (int it = 0; it < Q; ++it) {
+            const int f = it * 64 + lane, r = f / Q, q = f % Q, col = c0 + 4 * q;
+            const long rr = row0 + r < rows ? row0 + r : rows - 1;
+            const float *rp = c + (size_t)rr * n;
+            if (vec && col + 3 < n) {
+                const float4 v = *reinterpret_cast<const float4 *>(rp + col);
+                t[r][4 * q] = v.x;
+                t[r][4 * q + 1] = v.y;
+                t[r][4 * q + 2] = v.z;
+                t[r][4 * q + 3] = v.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) t[r][4 * q + e] = rp[col + e < n ? col + e : n - 1];
+            }
+        }
+    };
+    // columns [c0, c0 + B) of the wave's rows <- tile (only columns < n, rows < rows)
+    auto stage_out = [&](int c0) {
+#pragma unroll
+        for (int it = 0; it < Q; ++it) {
+            const int f = it * 64 + lane, r = f / Q, q = f % Q, col = c0 + 4 * q;
+            if (row0 + r >= rows) continue;
+            float *rp = c + (size_t)(row0 + r) * n;
+            if (vec && col + 3 < n) {
+                *reinterpret_cast<float4 *>(rp + col) =
+                    make_float4(t[r][4 * q], t[r][4 * q + 1], t[r][4 * q + 2], t[r][4 * q + 3]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (col + e < n) rp[col + e] = t[r][4 * q + e];
+            }
+        }
+    };
+    const float z1 = -0.267949192431122706f;  // sqrt(3) - 2
+    if (n == 1) return;                       // a one-sample row is its own coefficient
+    float w[W];
+    float cprev = 0.f;
+    // first window: columns [0, W) in H tiles; the causal start from the
+    // first tile (every index of the truncated mirrored sum is below 24)
+    const int period = 2 * n - 2;
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+        __syncthreads();
+        stage_in(h * B);
+        __syncthreads();
+        if (h == 0) {
+            float c0 = 0.f, zk = 1.f;
+#pragma unroll
+            for (int m = 0; m < kPrefInitTerms; ++m) {
+                const int mm = m < period ? m : 0;
+                const float x = t[lane][mm < n ? mm : period - mm];
+                c0 = fmaf(m < period ? x : 0.f, zk, c0);
+                zk *= z1;
+            }
+            float zp = 1.f;
+            for (int m = 0; m < period && m < 64; ++m) zp *= z1;
+            cprev = c0 / (1.f - zp);
+            w[0] = cprev;
+        }
+#pragma unroll
+        for (int k = (h == 0 ? 1 : 0); k < B; ++k) {
+            if (h * B + k < W) {
+                cprev = fmaf(z1, cprev, t[lane][k]);  // past n - 1: unused values
+                w[h * B + k] = cprev;
+            }
+        }
+    }
+    int a = 0;
+    while (n - a > W) {
+        float cn = 0.f;
+        __syncthreads();  // the tile's last reads are done: outputs go straight into it
+#pragma unroll
+        for (int k = W - 1; k >= 0; --k) {
+            cn = z1 * (cn - w[k]);
+            if (k < B) t[lane][k] = 6.f * cn;
+        }
+        __syncthreads();
+        stage_out(a);
+#pragma unroll
+        for (int k = 0; k < D; ++k) w[k] = w[B + k];
+        __syncthreads();
+        stage_in(a + W);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            cprev = fmaf(z1, cprev, t[lane][k]);
+            w[D + k] = cprev;
+        }
+        a += B;
+    }
+    // last window: v = n - a values (2 <= v <= W), exact end condition
+    const int v = n - a;
+    float last = 0.f, prev = 0.f;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        if (k == v - 1) last = w[k];
+        if (k == v - 2) prev = w[k];
+    }
+    float cn = z1 / (z1 * z1 - 1.f) * fmaf(z1, prev, last);
+    // backwards, one tile of B outputs at a time (the highest columns first)
+#pragma unroll
+    for (int h = H - 1; h >= 0; --h) {
+        __syncthreads();
+#pragma unroll
+        for (int k = B - 1; k >= 0; --k) {
+            if (h * B + k < W && h * B + k < v - 1) cn = z1 * (cn - w[h * B + k]);
+            t[lane][k] = 6.f * cn;  // columns at or past n are not stored
+        }
+        __syncthreads();
+        stage_out(a + h * B);  // stores only columns < n
+    }
+}
+
 __global__ void k_prefilter_init(unsigned *w) {
     if (threadIdx.x == 0) {
         w[0] = 0u;           // non-finite flag
@@ -190,8 +325,8 @@ inline hipError_t launch_prefilter3d(float *c, const int N[3], float shift, unsi
     const long ly = (long)N[0] * N[2], lx = (long)N[1] * N[2], lz = (long)N[0] * N[1];
     hipLaunchKernelGGL((k_prefilter<1, kPrefB, kPrefDyz>), dim3((unsigned)((ly + kPrefBlock - 1) / kPrefBlock)), dim3(kPrefBlock), 0, s,
                        c, N[0], N[1], N[2], shift, w, nullptr);
-    hipLaunchKernelGGL((k_prefilter<0, kPrefB, kPrefDx>), dim3((unsigned)((lx + kPrefBlock - 1) / kPrefBlock)), dim3(kPrefBlock), 0, s,
-                       c, N[0], N[1], N[2], 0.f, nullptr, nullptr);
+    hipLaunchKernelGGL((k_prefilter_rows<kPrefB, kPrefDx>), dim3((unsigned)((lx + kPrefRowBlock - 1) / kPrefRowBlock)),
+                       dim3(kPrefRowBlock), 0, s, c, N[0], lx);
     hipLaunchKernelGGL((k_prefilter<2, kPrefB, kPrefDyz>), dim3((unsigned)((lz + kPrefBlock - 1) / kPrefBlock)), dim3(kPrefBlock), 0, s,
                        c, N[0], N[1], N[2], 0.f, nullptr, w + 1);
     return hipGetLastError();
