@@ -319,11 +319,10 @@ Scene<V>::Scene(Float ior, const V<Float> &blockL, const V<Float> &blockR, const
             sensor_lens_origin, sensor_lens_aperture, sensor_lens_focalLength, sensor_lens_active, f_u, speed_u,
             n_o, n_max, n_clip, phi_min, phi_max, mode, axis_uz, axis_ux, p_u, er_stepsize, tol, rrWeight,
             precision, EgapEndLocX, SgapBeginLocX, useInitializationHack) {
-    // spline.h:88-162 + rif_interpolated.h:101-104: read the VOL grid of n
-    // and prefilter it into B-spline coefficients
-    spline::VolGrid g = spline::readVol(rifgridFile);
-    m_spline = g;
-    m_spline.data = spline::prefilter3d(g.data, g.N);
+    // spline.h:88-162 + rif_interpolated.h:101-104: read the VOL grid of n;
+    // the prefilter into B-spline coefficients runs on the GPU when the scene
+    // is uploaded (simpler_scene_set_spline_samples)
+    m_spline_samples = spline::readVol(rifgridFile);
     m_desc.field_kind = SIMPLER_FIELD_SPLINE;
 }
 
@@ -446,6 +445,15 @@ Float Scene<V>::getFresnelTrans() const {
     return 1.0f;  // Fresnel compiled out in the reference build (scene.h:157-159)
 }
 
+template <template <typename> class V>
+const spline::VolGrid &Scene<V>::spline_coeffs() const {
+    if (m_spline_coeffs.data.size() != m_spline_samples.data.size()) {
+        m_spline_coeffs = m_spline_samples;
+        m_spline_coeffs.data = spline::prefilter3d(m_spline_samples.data, m_spline_samples.N);
+    }
+    return m_spline_coeffs;
+}
+
 template class Scene<tvec::TVector3>;
 
 }  // namespace scn
@@ -503,8 +511,8 @@ void Renderer<V>::renderChunk(image::SmallImage &img0, const med::Medium &medium
                                         (int)scene.source_coeff().size(), scene.source_coeff().data()),
               "sources upload");
     if (d.field_kind == SIMPLER_FIELD_SPLINE) {
-        const spline::VolGrid &g = scene.spline_coeffs();
-        check(ctx, simpler_scene_set_spline(sc, g.data.data(), g.N, g.xmin, g.xmax), "spline upload");
+        const spline::VolGrid &g = scene.spline_samples();
+        check(ctx, simpler_scene_set_spline_samples(sc, g.data.data(), 0, g.N, g.xmin, g.xmax), "spline upload");
     }
     check(ctx,
           simpler_render_host(sc, numPhotons, nTotal, m_seed, (uint64_t)offset, img0.data(), img0.get_x_res(),

@@ -225,7 +225,10 @@ public:
     const std::vector<float> &source_amp() const { return m_src_amp; }
     const std::vector<float> &source_phase() const { return m_src_phase; }
     const std::vector<float> &source_coeff() const { return m_src_coeff; }
-    const spline::VolGrid &spline_coeffs() const { return m_spline; }
+    // the VOL grid's node values, and its B-spline coefficients (computed on
+    // the host on first use; renders prefilter on the GPU instead)
+    const spline::VolGrid &spline_samples() const { return m_spline_samples; }
+    const spline::VolGrid &spline_coeffs() const;
     uint64_t revision() const { return m_revision; }
 
 private:
@@ -254,7 +257,8 @@ private:
     bool m_active1 = false, m_active2 = false;
     tvec::Vec3f m_center1, m_center2;
     std::vector<float> m_src_pos, m_src_amp, m_src_phase, m_src_coeff;
-    spline::VolGrid m_spline;
+    spline::VolGrid m_spline_samples;
+    mutable spline::VolGrid m_spline_coeffs;
     uint64_t m_revision = 0;
 };
 

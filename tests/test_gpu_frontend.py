@@ -132,5 +132,11 @@ def test_renderer3d_sample_spline_grid_render(gpu_ctx, tmp_path):
     assert r.returncode == 0, r.stderr
     run_exe(tmp_path, [build_host.frontend_path("renderer3d_sample")] + args)
     img = read_pfm3d(str(tmp_path / "spl.pfm3d"))
-    coeff = scene_pybind.prefilter3d(grid).reshape(-1)
-    compare(img, engine_render(gpu_ctx, d, tex, spline=(coeff, list(dims), list(lo), list(hi))))
+    # the front-end prefilters the VOL grid on the GPU, as set_spline_samples does
+    sc = engine.Scene(gpu_ctx, desc_from(d), tex)
+    sc.set_spline_samples(grid.reshape(-1), list(dims), list(lo), list(hi))
+    ref, _ = sc.render(d["num_photons"], (d["x_res"], d["y_res"], d["z_res"]), seed=0)
+    compare(img, ref.double().cpu().numpy())
+    # and its coefficients are the host prefilter's to FP32 accuracy
+    host = scene_pybind.prefilter3d(grid).reshape(-1)
+    assert np.abs(sc.spline_coeffs() - host).max() <= 1e-6 * np.abs(host).max()
