@@ -110,9 +110,12 @@ __device__ __forceinline__ void field_eval(const DevScene &S, const FieldP &F, f
 // reference scene.cpp:205-257 (axis = +-x component form) or the same
 // gradient for a general axis: grad n = A s [J'(kr) k cos(m phi) rho/r
 //                                           - J m sin(m phi) (uz x rho)/r^2]
-template <>
-__device__ __forceinline__ void field_eval<kBessel>(const DevScene &S, const FieldP &F, f3 p, float scaling,
-                                                    float, float &n, f3 &g) {
+// MM >= 0: the order m known at compile time (m = 0 drops the angular
+// terms; the angle-addition recurrence unrolls); MM < 0: m = F.mode.
+template <int MM>
+__device__ __forceinline__ void bessel_eval(const DevScene &S, const FieldP &F, f3 p, float scaling, float &n,
+                                            f3 &g) {
+    const int mode = MM >= 0 ? MM : F.mode;
     if (p.x > S.egap || p.x < S.sgap) {
         n = S.n_o;
         g = mk3(0.f, 0.f, 0.f);
@@ -127,13 +130,16 @@ __device__ __forceinline__ void field_eval<kBessel>(const DevScene &S, const Fie
     float r = r2 * ir;
     // cos(m phi), sin(m phi) with phi = atan2(rho.uy, rho.ux) by the
     // angle-addition recurrence (atan2(0,0) = 0 at r = 0)
-    const float c1 = nz ? dot(rho, S.ux) * ir : 1.0f;
-    const float s1 = nz ? dot(rho, S.uy) * ir : 0.0f;
     float cm = 1.0f, sm = 0.0f;
-    for (int k = 0; k < F.mode; ++k) {
-        const float cn = cm * c1 - sm * s1;
-        sm = sm * c1 + cm * s1;
-        cm = cn;
+    if (MM != 0) {
+        const float c1 = nz ? dot(rho, S.ux) * ir : 1.0f;
+        const float s1 = nz ? dot(rho, S.uy) * ir : 0.0f;
+#pragma unroll
+        for (int k = 0; k < (MM >= 0 ? MM : mode); ++k) {
+            const float cn = cm * c1 - sm * s1;
+            sm = sm * c1 + cm * s1;
+            cm = cn;
+        }
     }
     const float A = F.amp * scaling;
     float J, dJ;
@@ -155,13 +161,23 @@ __device__ __forceinline__ void field_eval<kBessel>(const DevScene &S, const Fie
     bessel_tab(F, F.k_r * r, J, dJ);
     if (!clamp) n = S.n_o + A * J * cm;
     const float cr = A * dJ * F.k_r * ir * cm;
-    const float ctan = A * J * (float)F.mode * sm * ir * ir;
+    if (MM == 0) {
+        g = S.general_axis ? cr * rho : mk3(0.f, cr * rho.y, cr * rho.z);
+        return;
+    }
+    const float ctan = A * J * (float)mode * sm * ir * ir;
     if (S.general_axis) {
         const f3 tp = cross(S.uz, rho);
         g = mk3(cr * rho.x - ctan * tp.x, cr * rho.y - ctan * tp.y, cr * rho.z - ctan * tp.z);
     } else {
         g = mk3(0.f, cr * rho.y - ctan * rho.z, cr * rho.z + ctan * rho.y);
     }
+}
+
+template <>
+__device__ __forceinline__ void field_eval<kBessel>(const DevScene &S, const FieldP &F, f3 p, float scaling,
+                                                    float, float &n, f3 &g) {
+    bessel_eval<-1>(S, F, p, scaling, n, g);
 }
 
 // reference scene.cpp:140-195, generalised (see simpler.h)

@@ -228,7 +228,7 @@ __device__ __forceinline__ bool inside_block(const DevScene &S, f3 p) {
 // steps_left is the caller's register copy of h.steps_left; returns true when
 // the segment ended (the caller then sets ST_SEG_END), so the loop state
 // never goes through memory.
-template <int FIELD, bool LOG = false>
+template <int FIELD, bool LOG = false, int MM = -1>
 __device__ __forceinline__ bool march_step(Hot &h, uint32_t &steps_left, const DevScene &S, const FieldP &F,
                                            float Lcut) {
     const float hs = h.h;
@@ -244,7 +244,10 @@ __device__ __forceinline__ bool march_step(Hot &h, uint32_t &steps_left, const D
     const f3 p1 = mk3(px, py, pz);
     float n1;
     f3 g1;
-    field_eval<FIELD>(S, F, p1, h.scaling, h.tau, n1, g1);
+    if constexpr (FIELD == kBessel)
+        bessel_eval<MM>(S, F, p1, h.scaling, n1, g1);
+    else
+        field_eval<FIELD>(S, F, p1, h.scaling, h.tau, n1, g1);
     const f3 dv1 = dvh + hh * g1;
     ++h.steps;
     const int lev = S.precision - h.prec;
@@ -792,7 +795,9 @@ namespace simpler {
 // the few spills fall in the event code), measured 3-6 % faster than 4 waves;
 // the spline and point-source kernels keep the registers their evaluation
 // needs (forcing 96 VGPRs spills inside their march step).
-template <int FIELD, int MODE, bool BATCH, bool JLDS = false>
+// MM: the Bessel order when it is known at launch (single-scene renders of
+// m = 0 or 2), -1 otherwise.
+template <int FIELD, int MODE, bool BATCH, bool JLDS = false, int MM = -1>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FIELD == kBessel ? 5 : (FIELD == kSpline ? 3 : 1))))
 k_render(DevScene S, RenderArgs A) {
     __shared__ Cold cold[kBlock];
@@ -856,7 +861,7 @@ k_render(DevScene S, RenderArgs A) {
         bool m = h.state == ST_MARCH;
         uint32_t sl = h.steps_left;
         for (int k = 0; k < A.spc; ++k)
-            if (m) m = !march_step<FIELD>(h, sl, S, F, Lcut);
+            if (m) m = !march_step<FIELD, false, MM>(h, sl, S, F, Lcut);
         h.steps_left = sl;
         if (h.state == ST_MARCH && !m) h.state = ST_SEG_END;
     }
@@ -1183,7 +1188,11 @@ static void launch_render(simpler_ctx *ctx, int field, const DevScene &S, const 
     const dim3 grid(blocks), block(ctx->block);
     switch (field) {
         case SIMPLER_FIELD_BESSEL:
-            if (S.jtab_n <= kJtabLds)
+            if (MODE == M_RENDER && S.jtab_n <= kJtabLds && S.mode == 0)
+                hipLaunchKernelGGL((k_render<kBessel, M_RENDER, false, true, 0>), grid, block, 0, ctx->cur, S, A);
+            else if (MODE == M_RENDER && S.jtab_n <= kJtabLds && S.mode == 2)
+                hipLaunchKernelGGL((k_render<kBessel, M_RENDER, false, true, 2>), grid, block, 0, ctx->cur, S, A);
+            else if (S.jtab_n <= kJtabLds)
                 hipLaunchKernelGGL((k_render<kBessel, MODE, false, true>), grid, block, 0, ctx->cur, S, A);
             else
                 hipLaunchKernelGGL((k_render<kBessel, MODE, false>), grid, block, 0, ctx->cur, S, A);
