@@ -20,6 +20,7 @@ namespace simpler {
 // cuda_renderer.cu:48).
 struct DevScene {
     f3 blockL, blockR;
+    f3 inL, inR;  // blockL - eps, blockR + eps (rounded from double): the inside test's bounds
     float ior;
     // source
     f3 light_origin;

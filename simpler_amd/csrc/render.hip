@@ -212,9 +212,13 @@ __device__ __forceinline__ void end_segment(Hot &h) {
     h.Ll = 0.f;
 }
 
+// block.cpp:11-23: p - blockL > -eps and blockR - p > -eps per axis, as
+// comparisons with the bounds blockL - eps and blockR + eps.  Near a face the
+// reference's difference is exact (Sterbenz), so the two tests differ only
+// for p within half an ulp of the rounded bound.
 __device__ __forceinline__ bool inside_block(const DevScene &S, f3 p) {
-    return (p.x - S.blockL.x > -kEps) && (S.blockR.x - p.x > -kEps) && (p.y - S.blockL.y > -kEps) &&
-           (S.blockR.y - p.y > -kEps) && (p.z - S.blockL.z > -kEps) && (S.blockR.z - p.z > -kEps);
+    return (p.x > S.inL.x) && (p.x < S.inR.x) && (p.y > S.inL.y) && (p.y < S.inR.y) && (p.z > S.inL.z) &&
+           (p.z < S.inR.z);
 }
 
 // One ER step of traceTillBlock (scene.cpp:395-418) with er_step
@@ -228,7 +232,7 @@ __device__ __forceinline__ bool inside_block(const DevScene &S, f3 p) {
 // steps_left is the caller's register copy of h.steps_left; returns true when
 // the segment ended (the caller then sets ST_SEG_END), so the loop state
 // never goes through memory.
-template <int FIELD, bool LOG = false, int MM = -1>
+template <int FIELD, bool LOG = false, int MM = -1, bool COUNT = true>
 __device__ __forceinline__ bool march_step(Hot &h, uint32_t &steps_left, const DevScene &S, const FieldP &F,
                                            float Lcut) {
     const float hs = h.h;
@@ -249,7 +253,7 @@ __device__ __forceinline__ bool march_step(Hot &h, uint32_t &steps_left, const D
     else
         field_eval<FIELD>(S, F, p1, h.scaling, h.tau, n1, g1);
     const f3 dv1 = dvh + hh * g1;
-    ++h.steps;
+    if (COUNT) ++h.steps;
     const int lev = S.precision - h.prec;
     const bool reject = !inside_block(S, p1) || (h.travelled + hs) > h.seg_dist;
     const bool stop = reject && (--h.prec < 0);  // past the face / distance (scene.cpp:405-407)
@@ -818,6 +822,7 @@ k_render(DevScene S, RenderArgs A) {
     h.dead = false;
     Lane L{h, cold[threadIdx.x], v, S, A, cnt, nullptr, &mcache[threadIdx.x]};
     const int lane = __lane_id();
+    unsigned long long wave_steps = 0;  // ER steps of the wave (uniform)
     while (true) {
         // fetch: one atomic per wave for all lanes that need a photon
         const unsigned long long need = __ballot(h.state == ST_NEED);
@@ -860,16 +865,16 @@ k_render(DevScene S, RenderArgs A) {
         const float Lcut = v.Lcut;
         bool m = h.state == ST_MARCH;
         uint32_t sl = h.steps_left;
-        for (int k = 0; k < A.spc; ++k)
-            if (m) m = !march_step<FIELD, false, MM>(h, sl, S, F, Lcut);
+        for (int k = 0; k < A.spc; ++k) {
+            wave_steps += __popcll(__ballot(m));  // one ER step per marching lane
+            if (m) m = !march_step<FIELD, false, MM, false>(h, sl, S, F, Lcut);
+        }
         h.steps_left = sl;
         if (h.state == ST_MARCH && !m) h.state = ST_SEG_END;
     }
     // counters: ER steps reduced over the wave, events from LDS, one atomic
     // per counter per wave / block
-    unsigned long long st = h.steps;
-    for (int off = 32; off > 0; off >>= 1) st += __shfl_xor(st, off);
-    if (lane == 0 && st) atomicAdd(A.counters + C_STEPS, st);
+    if (lane == 0 && wave_steps) atomicAdd(A.counters + C_STEPS, wave_steps);
     __syncthreads();
     if (threadIdx.x < C_N && threadIdx.x != C_STEPS && cnt[threadIdx.x])
         atomicAdd(A.counters + threadIdx.x, (unsigned long long)cnt[threadIdx.x]);
@@ -1004,6 +1009,10 @@ static DevScene make_dev_scene(const simpler_scene *sc, int64_t n_total, uint64_
     DevScene S;
     std::memset(&S, 0, sizeof(S));
     S.blockL = h3(d.block_l);
+    S.inL = f3{(float)((double)d.block_l[0] - kEps), (float)((double)d.block_l[1] - kEps),
+               (float)((double)d.block_l[2] - kEps)};
+    S.inR = f3{(float)((double)d.block_r[0] + kEps), (float)((double)d.block_r[1] + kEps),
+               (float)((double)d.block_r[2] + kEps)};
     S.blockR = h3(d.block_r);
     S.ior = d.ior;
     S.light_origin = h3(d.light_origin);
