@@ -113,10 +113,12 @@ __device__ __forceinline__ void field_eval(const DevScene &S, const FieldP &F, f
 //                                           - J m sin(m phi) (uz x rho)/r^2]
 // MM >= 0: the order m known at compile time (m = 0 drops the angular
 // terms; the angle-addition recurrence unrolls); MM < 0: m = F.mode.
-template <int MM>
+// GA >= 0: the axis form known at compile time; GA < 0: S.general_axis.
+template <int MM, int GA = -1>
 __device__ __forceinline__ void bessel_eval(const DevScene &S, const FieldP &F, f3 p, float scaling, float &n,
                                             f3 &g) {
     const int mode = MM >= 0 ? MM : F.mode;
+    const bool general = GA >= 0 ? (GA != 0) : (S.general_axis != 0);
     if (p.x > S.egap || p.x < S.sgap) {
         n = S.n_o;
         g = mk3(0.f, 0.f, 0.f);
@@ -150,7 +152,7 @@ __device__ __forceinline__ void bessel_eval(const DevScene &S, const FieldP &F, 
         // (reference scene.cpp:209-214 vs 229-234)
         bessel_tab(F, F.k_r * r, J, dJ);
         n = S.n_o + A * J * cm;
-        if (S.general_axis) {
+        if (general) {
             rho = kEps * (S.ux + S.uy);
         } else {
             rho.y = kEps;
@@ -163,11 +165,11 @@ __device__ __forceinline__ void bessel_eval(const DevScene &S, const FieldP &F, 
     if (!clamp) n = S.n_o + A * J * cm;
     const float cr = A * dJ * F.k_r * ir * cm;
     if (MM == 0) {
-        g = S.general_axis ? cr * rho : mk3(0.f, cr * rho.y, cr * rho.z);
+        g = general ? cr * rho : mk3(0.f, cr * rho.y, cr * rho.z);
         return;
     }
     const float ctan = A * J * (float)mode * sm * ir * ir;
-    if (S.general_axis) {
+    if (general) {
         const f3 tp = cross(S.uz, rho);
         g = mk3(cr * rho.x - ctan * tp.x, cr * rho.y - ctan * tp.y, cr * rho.z - ctan * tp.z);
     } else {

@@ -232,7 +232,7 @@ __device__ __forceinline__ bool inside_block(const DevScene &S, f3 p) {
 // steps_left is the caller's register copy of h.steps_left; returns true when
 // the segment ended (the caller then sets ST_SEG_END), so the loop state
 // never goes through memory.
-template <int FIELD, bool LOG = false, int MM = -1, bool COUNT = true>
+template <int FIELD, bool LOG = false, int MM = -1, bool COUNT = true, int GA = -1>
 __device__ __forceinline__ bool march_step(Hot &h, uint32_t &steps_left, const DevScene &S, const FieldP &F,
                                            float Lcut) {
     const float hs = h.h;
@@ -249,7 +249,7 @@ __device__ __forceinline__ bool march_step(Hot &h, uint32_t &steps_left, const D
     float n1;
     f3 g1;
     if constexpr (FIELD == kBessel)
-        bessel_eval<MM>(S, F, p1, h.scaling, n1, g1);
+        bessel_eval<MM, GA>(S, F, p1, h.scaling, n1, g1);
     else
         field_eval<FIELD>(S, F, p1, h.scaling, h.tau, n1, g1);
     const f3 dv1 = dvh + hh * g1;
@@ -800,8 +800,9 @@ namespace simpler {
 // the spline and point-source kernels keep the registers their evaluation
 // needs (forcing 96 VGPRs spills inside their march step).
 // MM: the Bessel order when it is known at launch (single-scene renders of
-// m = 0 or 2), -1 otherwise.
-template <int FIELD, int MODE, bool BATCH, bool JLDS = false, int MM = -1>
+// m = 0 or 2), -1 otherwise; GA: the axis form then (1 general, 0 the
+// reference's axis = x form).
+template <int FIELD, int MODE, bool BATCH, bool JLDS = false, int MM = -1, int GA = -1>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FIELD == kBessel ? 5 : (FIELD == kSpline ? 3 : 1))))
 k_render(DevScene S, RenderArgs A) {
     __shared__ Cold cold[kBlock];
@@ -867,7 +868,7 @@ k_render(DevScene S, RenderArgs A) {
         uint32_t sl = h.steps_left;
         for (int k = 0; k < A.spc; ++k) {
             wave_steps += __popcll(__ballot(m));  // one ER step per marching lane
-            if (m) m = !march_step<FIELD, false, MM, false>(h, sl, S, F, Lcut);
+            if (m) m = !march_step<FIELD, false, MM, false, GA>(h, sl, S, F, Lcut);
         }
         h.steps_left = sl;
         if (h.state == ST_MARCH && !m) h.state = ST_SEG_END;
@@ -1197,11 +1198,16 @@ static void launch_render(simpler_ctx *ctx, int field, const DevScene &S, const 
     const dim3 grid(blocks), block(ctx->block);
     switch (field) {
         case SIMPLER_FIELD_BESSEL:
-            if (MODE == M_RENDER && S.jtab_n <= kJtabLds && S.mode == 0)
-                hipLaunchKernelGGL((k_render<kBessel, M_RENDER, false, true, 0>), grid, block, 0, ctx->cur, S, A);
-            else if (MODE == M_RENDER && S.jtab_n <= kJtabLds && S.mode == 2)
-                hipLaunchKernelGGL((k_render<kBessel, M_RENDER, false, true, 2>), grid, block, 0, ctx->cur, S, A);
-            else if (S.jtab_n <= kJtabLds)
+            if (MODE == M_RENDER && S.jtab_n <= kJtabLds && (S.mode == 0 || S.mode == 2)) {
+                if (S.mode == 0 && S.general_axis)
+                    hipLaunchKernelGGL((k_render<kBessel, M_RENDER, false, true, 0, 1>), grid, block, 0, ctx->cur, S, A);
+                else if (S.mode == 0)
+                    hipLaunchKernelGGL((k_render<kBessel, M_RENDER, false, true, 0, 0>), grid, block, 0, ctx->cur, S, A);
+                else if (S.general_axis)
+                    hipLaunchKernelGGL((k_render<kBessel, M_RENDER, false, true, 2, 1>), grid, block, 0, ctx->cur, S, A);
+                else
+                    hipLaunchKernelGGL((k_render<kBessel, M_RENDER, false, true, 2, 0>), grid, block, 0, ctx->cur, S, A);
+            } else if (S.jtab_n <= kJtabLds)
                 hipLaunchKernelGGL((k_render<kBessel, MODE, false, true>), grid, block, 0, ctx->cur, S, A);
             else
                 hipLaunchKernelGGL((k_render<kBessel, MODE, false>), grid, block, 0, ctx->cur, S, A);
