@@ -51,50 +51,73 @@ def algorithmic_flops(st, mode):
 
 
 def gpu_bench(args):
+    """The timed workload.  Each mode renders its chunks on a context of its
+    own (its own HIP stream), so the two modes' launches overlap: a launch's
+    last waves (the photons that happen to run longest) share the GPU with
+    the other mode's launch instead of leaving it idle.  Before a context
+    launches its next chunk it reads the previous launch's event counters,
+    which waits for that launch only."""
     import torch
 
     from simpler_amd import engine, scenes
 
-    ctx = engine.Context(0)
-    autotuned = None
-    if args.tune:
-        ctx.set_tuning(*[int(x) for x in args.tune.split(",")])
     B = args.photons_per_step
     modes = [int(m) for m in args.modes.split(",")]
+    ctxs = {m: engine.Context(0, use_torch_stream=(i == 0)) for i, m in enumerate(modes)}
+    autotuned = None
+    if args.tune:
+        for c in ctxs.values():
+            c.set_tuning(*[int(x) for x in args.tune.split(",")])
     work = {}
     for m in modes:
         d, tex, res = scenes.bessel_cube_full(m)
-        sc = engine.Scene(ctx, d, tex)
+        sc = engine.Scene(ctxs[m], d, tex)
         X, Y, Z = res
         img = torch.zeros((Z, Y, X), dtype=torch.float32, device="cuda")
         work[m] = (sc, img, res)
     n_tune = args.photons_per_step if args.autotune_photons < 0 else args.autotune_photons
     if not args.tune and n_tune > 0:  # the autotuner picks the launch shape on the workload itself
         autotuned = work[modes[0]][0].autotune(n_tune, work[modes[0]][2])
+        shape = ctxs[modes[0]].get_tuning()
+        for c in ctxs.values():
+            c.set_tuning(shape["block_size"], shape["steps_per_chunk"], shape["blocks_per_cu"])
         eprint(f"autotune: {autotuned}")
-    stream = torch.cuda.current_stream()
+    streams = {m: ctxs[m].stream() for m in modes}
+    pending = {m: None for m in modes}  # (events) of the launch in flight per mode
+    recs = []
 
-    def step(k, record):
-        out = []
+    def collect(m):
+        if pending[m] is None:
+            return
+        e0, e1, timed = pending[m]
+        st = ctxs[m].read_stats()  # waits for this mode's launch only
+        if timed:
+            recs.append((m, st, e0, e1))
+        pending[m] = None
+
+    def step(k, timed):
         for m in modes:
+            collect(m)
             sc, img, res = work[m]
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            _, st = sc.render(B, res, seed=1, offset=k * B, n_total=N_TOTAL, image=img, accumulate=True)
-            e1.record(stream)
-            out.append((m, st, e0, e1))
-        return out
+            e0.record(streams[m])
+            sc.render(B, res, seed=1, offset=k * B, n_total=N_TOTAL, image=img, accumulate=True, stats=False)
+            e1.record(streams[m])
+            pending[m] = (e0, e1, timed)
 
     for w in range(args.warmup):
         step(w, False)
+        for m in modes:
+            collect(m)
         eprint(f"warmup {w + 1}/{args.warmup} done")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    recs = []
     for k in range(args.steps):
-        recs += step(args.warmup + k, True)
-        eprint(f"step {k + 1}/{args.steps} done")
+        step(args.warmup + k, True)
+        eprint(f"step {k + 1}/{args.steps} launched")
+    for m in modes:
+        collect(m)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     per_mode = {}
@@ -113,15 +136,18 @@ def gpu_bench(args):
         tot_ms += ms
         tot_steps += st["er_steps"]
     photons = sum(pm["photons"] for pm in per_mode.values())
-    res = dict(wall=wall, photons=photons, er_steps=tot_steps, kernel_ms=tot_ms, flops=tot_flops,
-               per_mode={str(m): dict(photon_paths_per_s=pm["photons"] / (pm["kernel_ms"] / 1e3),
-                                      er_steps_per_s=pm["er_steps"] / (pm["kernel_ms"] / 1e3),
+    # GPU time of the timed launches: first launch start to last launch end
+    first = recs[0][2]
+    span_ms = max(first.elapsed_time(e1) for _, _, _, e1 in recs)
+    res = dict(wall=wall, photons=photons, er_steps=tot_steps, kernel_ms=tot_ms, flops=tot_flops, span_ms=span_ms,
+               per_mode={str(m): dict(photons=pm["photons"], er_steps_per_photon=pm["er_steps"] / pm["photons"],
                                       avg_launch_ms=pm["kernel_ms"] / pm["launches"],
-                                      er_steps_per_photon=pm["er_steps"] / pm["photons"],
-                                      tflops=pm["flops"] / (pm["kernel_ms"] / 1e3) / 1e12)
+                                      launch_note="launch start to end on its stream; the modes' launches overlap")
                          for m, pm in per_mode.items()},
-               tuning=dict(ctx.get_tuning(), autotuned=autotuned is not None), launches=len(recs))
-    ctx.close()
+               tuning=dict(ctxs[modes[0]].get_tuning(), autotuned=autotuned is not None), launches=len(recs))
+    for m in modes:
+        work[m][0].close()
+        ctxs[m].close()
     return res
 
 
@@ -416,7 +442,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--photons-per-step", type=int, default=1 << 22)
+    ap.add_argument("--photons-per-step", type=int, default=1 << 23)
     ap.add_argument("--modes", default="0,2")
     ap.add_argument("--tune", default="", help="block_size,steps_per_chunk,blocks_per_cu (skips the autotuner)")
     ap.add_argument("--autotune-photons", type=int, default=-1,
@@ -462,7 +488,10 @@ def main():
         sys.exit(2)
     g = gpu_bench(args)
     value = g["photons"] / g["wall"]
-    achieved = g["flops"] / (g["kernel_ms"] / 1e3) / 1e12
+    # the render kernel keeps the GPU busy over the timed launches (the two
+    # modes' launches overlap), so its rate is the algorithmic flops of all
+    # timed launches over their GPU time (first start to last end, HIP events)
+    achieved = g["flops"] / (g["span_ms"] / 1e3) / 1e12
     out = {
         "metric": METRIC,
         "value": value,
@@ -491,7 +520,8 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": TRAFFIC_BYTES_PER_PHOTON * args.photons_per_step,
-                     "kernel": "k_render<Bessel> (persistent march + event kernel)",
+                     "kernel": "k_render<Bessel> (persistent march + event kernel), modes 0 and 2 on two streams",
+                     "gpu_time_ms": g["span_ms"], "launches": g["launches"],
                      "avg_launch_ms": g["kernel_ms"] / g["launches"]},
     }
     if not args.no_batch:

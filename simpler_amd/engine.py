@@ -137,6 +137,15 @@ class Context:
     def synchronize(self):
         self.check(self.L.simpler_ctx_synchronize(self.h))
 
+    def stream(self):
+        """The HIP stream the context runs on, as a torch stream (for events)."""
+        import torch
+        ptr = self.L.simpler_ctx_get_stream(self.h) or 0
+        cur = torch.cuda.current_stream(self.device)
+        if ptr == cur.cuda_stream:
+            return cur
+        return torch.cuda.ExternalStream(ptr, device=f"cuda:{self.device}")
+
     def set_tuning(self, block_size=0, steps_per_chunk=0, blocks_per_cu=0):
         self.check(self.L.simpler_ctx_set_tuning(self.h, block_size, steps_per_chunk, blocks_per_cu))
 
