@@ -170,6 +170,27 @@ def gaussian_spline_coeffs(n=64, half=0.005, n0=1.3333, max_dn=5e-5,
     return (n0 + sm).astype(np.float64), (n, n, n), (-half,) * 3, (half,) * 3
 
 
+def fus_sources_scene(res=(1024, 1024, 1)):
+    """The reference's renderings/settings/fus.json scene: a 12.7 mm slab
+    (x in [-6.35, 6.35] mm, y and z +-5 m), the two 100 x 51 arc lattices of
+    point sources (radius 50.8 mm, chord 12.7 mm, centres (0, 0, -50.8) and
+    (0, -50.8, 0) mm, phase pi each, n_scaling 2.15e-5, one harmonic), 2 MHz,
+    1500 m/s, transparent (sigma_t 0: the direct term only), 100 um ER step,
+    1 mm emitter and sensor, path length 0-100 m.  Returns (desc, texture,
+    res, sources) with the Disk texture at 101 x 101 texels."""
+    half = 0.00635
+    d = base_desc(half=half, step=1e-4, precision=4, sensor_plane=0.001, emitter_size=0.001,
+                  pl_range=(0.0, 100000.0))
+    d.set(field_kind=FIELD_SOURCES, f_u=2e6, speed_u=1500.0, n_o=1.3333, sigma_t=0.0, albedo=0.9, g=0.9,
+          use_direct=1, time_coupling=0.0, block_l=(-half, -5.0, -5.0), block_r=(half, 5.0, 5.0),
+          emitter_lens_aperture=0.3, emitter_lens_focal_length=0.3, sensor_lens_aperture=0.3,
+          sensor_lens_focal_length=0.3)
+    th = math.asin(0.0127 / (2 * 0.0508))
+    src = reference_arc_sources(0.0508, (0.0, 0.0, -0.0508), (0.0, -0.0508, 0.0), True, True, math.pi, math.pi,
+                                -th, th, 100, -0.0127 / 2, 0.0127 / 2, 51, 2.15e-5, 1)
+    return d, disk_texture(101), res, src
+
+
 def spline_cube(n=64, sigma_s=2000.0, g=0.9, step=5e-6, res=(64, 64, 32), seed=0):
     """Cube with a tricubic B-spline index field (node values from
     gaussian_spline_coeffs)."""

@@ -87,6 +87,38 @@ def test_transparent_per_ray_exit_matches_fp64(gpu_ctx, mode):
     assert ep < 1e-5 and ed < 1e-5 and eL < 1e-5
 
 
+def test_full_size_reference_sources_per_ray(gpu_ctx):
+    """The point-source field at the reference's own size (fus.json: 10,200
+    sources in two arc lattices, transparent slab): per-ray exit position,
+    direction and optical length against the FP64 oracle within 1e-5
+    relative, and the field's n and grad n at random points."""
+    import time
+    d, tex, _, src = scenes.fus_sources_scene()
+    assert len(src["amp"]) == 10200
+    sc = engine.Scene(gpu_ctx, d, tex, sources=src)
+    osc = O.OracleScene(d, tex, sources=src)
+    pts = _pts(n=256, seed=4, half=0.006)
+    n_o, g_o = osc.field(pts)
+    n_g, g_g = sc.field(torch.tensor(pts, dtype=torch.float32, device="cuda"))
+    n_g, g_g = n_g.double().cpu().numpy(), g_g.double().cpu().numpy()
+    assert np.abs(n_g - n_o).max() < 4e-7
+    assert np.abs(g_g - g_o).max() < 1e-4 * np.abs(g_o).max()
+    n = 48
+    p, dr, L, s = (t.double().cpu().numpy() for t in sc.trace_rays(n, seed=3))
+    p64, d64, L64, s64 = osc.trace_rays(n, seed=3, precision=64, threads=gpu_threads())
+    assert np.array_equal(s, s64) and (s == 1).all()
+    ep = np.abs(p - p64).max() / 0.00635
+    ed = np.abs(dr - d64).max()
+    eL = (np.abs(L - L64) / L64).max()
+    # the GPU's rate on the same scene, for the log
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    sc.trace_rays(1 << 16, seed=5)
+    torch.cuda.synchronize()
+    log(f"10,200 sources: pos {ep:.2e} dir {ed:.2e} L {eL:.2e}; GPU {(1 << 16) / (time.perf_counter() - t):.3g} rays/s")
+    assert ep < 1e-5 and ed < 1e-5 and eL < 1e-5
+
+
 def _batches(render, B):
     return np.stack([render(b) for b in range(B)])
 
