@@ -348,7 +348,7 @@ __device__ __forceinline__ void sub_rev_start(Lane &L, f3 lp, f3 lv, float mu, f
     const DevScene &S = L.S;
     float n;
     f3 g;
-    field_eval_h<FIELD>(S, L.v.f, h.p, h.scaling, h.tau, n, g, a.H);
+    field_eval_h<FIELD>(S, L.v->f, h.p, h.scaling, h.tau, n, g, a.H);
     h.n0 = n;
     h.g0 = g;
     a.lp = lp + ln * g;
@@ -372,13 +372,13 @@ __device__ bool nee_adj_start(Lane &L, f3 ps, float Lv, float val, f3 din, f3 dt
     const DevScene &S = L.S;
     float Wsum;
     f3 lps;
-    if (!splat_adj(S, L.A, L.v.image, ps, Lv, L.c.depth, val, Wsum, lps)) return false;
+    if (!splat_adj(S, L.A, L.v->image, ps, Lv, L.c.depth, val, Wsum, lps)) return false;
     a.F += (double)val * (double)Wsum;
     if (a.trunc_sub || a.trunc_main) {
         ++a.dropped;
         return false;
     }
-    const float g = L.v.g;
+    const float g = L.v->g;
     const float x = 1.f + g * g - 2.f * g * dot(din, dts);
     const float kh = Wsum * val * 3.f * g / x;  // dF / d(din . dts)
     f3 ldts = kh * din;
@@ -392,7 +392,7 @@ __device__ bool nee_adj_start(Lane &L, f3 ps, float Lv, float val, f3 din, f3 dt
     exit_adj(S, L.h.p, L.h.v, L.h.n0, lps, ldts, lrefr, lp, lv, ln);
     const float ml = len(L.c.mv);
     a.ldin = (ldin - dot(din, ldin) * din) / ml;
-    sub_rev_start<FIELD>(L, lp, lv, -L.v.sigma_t * Wsum * val, ln, RP_NEE);
+    sub_rev_start<FIELD>(L, lp, lv, -L.v->sigma_t * Wsum * val, ln, RP_NEE);
     return true;
 }
 
@@ -403,7 +403,7 @@ __device__ bool direct_adj_start(Lane &L, f3 ps, float Lv, float val) {
     const DevScene &S = L.S;
     float Wsum;
     f3 lps;
-    if (!splat_adj(S, L.A, L.v.image, ps, Lv, 0, val, Wsum, lps)) return false;
+    if (!splat_adj(S, L.A, L.v->image, ps, Lv, 0, val, Wsum, lps)) return false;
     a.F += (double)val * (double)Wsum;
     if (a.trunc_sub) {
         ++a.dropped;
@@ -412,7 +412,7 @@ __device__ bool direct_adj_start(Lane &L, f3 ps, float Lv, float val) {
     f3 lp, lv;
     float ln;
     exit_adj(S, L.h.p, L.h.v, L.h.n0, lps, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f), lp, lv, ln);
-    sub_rev_start<FIELD>(L, lp, lv, -L.v.sigma_t * Wsum * val, ln, RP_DIRECT);
+    sub_rev_start<FIELD>(L, lp, lv, -L.v->sigma_t * Wsum * val, ln, RP_DIRECT);
     return true;
 }
 

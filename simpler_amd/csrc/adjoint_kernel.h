@@ -14,7 +14,7 @@ __device__ __forceinline__ void rev_load(Lane &L, const float *base, int idx, f3
     rec_read(base, idx, p, v, lamp, lamv, cnt0, pat, kind);
     h.p = p;
     h.v = v;
-    field_eval_h<FIELD>(L.S, L.v.f, p, h.scaling, h.tau, h.n0, h.g0, a.H);
+    field_eval_h<FIELD>(L.S, L.v->f, p, h.scaling, h.tau, h.n0, h.g0, a.H);
     a.pa = 0.f;
     a.pb = mk3(0.f, 0.f, 0.f);
 }
@@ -45,7 +45,7 @@ __device__ void rev_end(Lane &L) {
     AdjState &a = *L.a;
     Hot &h = L.h;
     const DevScene &S = L.S;
-    const FieldP &F = L.v.f;
+    const FieldP &F = L.v->f;
     field_vjp<FIELD>(S, F, h.p, h.scaling, h.tau, a.pa, a.pb, a.G);
     f3 lamp, lamv;
     uint32_t cnt0, pat, kind;
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
     a.G.grid = A.grad;
     a.G.acc = sacc + threadIdx.x;
     a.G.stride = kBlock;
-    Lane L{h, cold[threadIdx.x], v, S, A, cnt, &a, nullptr};
+    Lane L{h, cold[threadIdx.x], &v, S, A, cnt, &a, nullptr};
     const int lane = __lane_id();
     while (true) {
         const unsigned long long need = __ballot(h.state == ST_NEED);

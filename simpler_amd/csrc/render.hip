@@ -46,6 +46,8 @@ enum { C_PHOT = 0, C_STEPS, C_SCAT, C_NEE, C_DIRECT, C_FACE, C_SEG, C_ABORT, C_N
 constexpr uint32_t kMaxFaceEvents = 1u << 20;  // per march; see the abort counter
 constexpr int kBlock = 256;
 constexpr int kJtabLds = 128;  // Bessel table intervals staged in LDS (8 KiB)
+constexpr int kJtabLdsBatch = 1024;  // float4 (16 KiB): a batch's per-order tables in LDS
+constexpr int kMaxMode = 7;          // highest Bessel order (check_desc)
 
 struct RenderArgs {
     float *image;
@@ -56,10 +58,14 @@ struct RenderArgs {
     float *rec_p, *rec_d, *rec_L;
     int *rec_s;
     int spc;
-    // batched launch: per-configuration parameters and photon prefix sums
+    // batched launch: per-configuration parameters and photon prefix sums,
+    // and the Bessel tables the configurations point into (staged in LDS
+    // when jtab_len float4 fit)
     const struct Var *cfgs;
     const unsigned long long *prefix;  // n_cfg + 1 entries
     int n_cfg;
+    const float4 *jtab_base;
+    int jtab_len;
     // adjoint: objective weights, parameter gradient, objective value,
     // lane-private segment logs, per-ray upstream gradients, counters
     const float *G;
@@ -124,7 +130,7 @@ __device__ __forceinline__ Var var_from(const DevScene &S, const RenderArgs &A) 
 struct Lane {
     Hot &h;
     Cold &c;
-    Var &v;
+    Var *v;  // the photon's configuration (the scene's, or a batch entry in global memory)
     const DevScene &S;
     const RenderArgs &A;
     uint32_t *cnt;          // block-level event counters in LDS
@@ -137,7 +143,7 @@ __device__ __forceinline__ void count(Lane &L, int k) { atomicAdd(L.cnt + k, 1u)
 __device__ __forceinline__ float draw_u(Lane &L) {
     Cold &c = L.c;
     uint32_t o[4];
-    philox4x32_10(c.pid_lo, c.pid_hi, c.draw >> 2, c.stream, L.v.seed_lo, L.v.seed_hi, o);
+    philox4x32_10(c.pid_lo, c.pid_hi, c.draw >> 2, c.stream, L.v->seed_lo, L.v->seed_hi, o);
     const uint32_t k = c.draw & 3u;
     const uint32_t w = k == 0 ? o[0] : (k == 1 ? o[1] : (k == 2 ? o[2] : o[3]));
     ++c.draw;
@@ -155,8 +161,8 @@ __device__ __forceinline__ void draw_k(Lane &L, float (&u)[K]) {
     Cold &c = L.c;
     const uint32_t d0 = c.draw, k0 = d0 & 3u;
     uint32_t o[8];
-    philox4x32_10(c.pid_lo, c.pid_hi, d0 >> 2, c.stream, L.v.seed_lo, L.v.seed_hi, o);
-    if (k0 + K > 4u) philox4x32_10(c.pid_lo, c.pid_hi, (d0 >> 2) + 1u, c.stream, L.v.seed_lo, L.v.seed_hi, o + 4);
+    philox4x32_10(c.pid_lo, c.pid_hi, d0 >> 2, c.stream, L.v->seed_lo, L.v->seed_hi, o);
+    if (k0 + K > 4u) philox4x32_10(c.pid_lo, c.pid_hi, (d0 >> 2) + 1u, c.stream, L.v->seed_lo, L.v->seed_hi, o + 4);
 #pragma unroll
     for (int i = 0; i < K; ++i) {
         const uint32_t j = k0 + (uint32_t)i;  // 0..7
@@ -170,7 +176,7 @@ __device__ __forceinline__ void draw_k(Lane &L, float (&u)[K]) {
 
 template <int FIELD>
 __device__ __forceinline__ void refresh_field(Lane &L) {
-    field_eval<FIELD>(L.S, L.v.f, L.h.p, L.h.scaling, L.h.tau, L.h.n0, L.h.g0);
+    field_eval<FIELD>(L.S, L.v->f, L.h.p, L.h.scaling, L.h.tau, L.h.n0, L.h.g0);
 }
 
 // n and grad n at the main-path point just restored: the saved values where
@@ -462,7 +468,7 @@ __device__ __forceinline__ void flight_start(Lane &L, float dist) {
 
 // photon.cpp:19-38
 __device__ void scatter_once(Lane &L) {
-    const float a = L.v.albedo;
+    const float a = L.v->albedo;
     if (!((a > 0.f) && ((a >= 1.f) || (draw_u(L) < a)))) {
         photon_end(L);
         return;
@@ -471,21 +477,21 @@ __device__ void scatter_once(Lane &L) {
     float u[3];  // HG (2), free path (1)
     draw_k<3>(L, u);
     const float mag = len(h.v);
-    const f3 d1 = hg_sample(L.v.g, h.v / mag, u[0], u[1]);
+    const f3 d1 = hg_sample(L.v->g, h.v / mag, u[0], u[1]);
     count(L, C_SCAT);
     h.v = mag * d1;
-    const float dist = -L.v.mfp * logf(u[2]);
+    const float dist = -L.v->mfp * logf(u[2]);
     flight_start(L, dist);
 }
 
 // photon.cpp:97-104: albedo test and first free flight
 __device__ void scatter_start(Lane &L) {
-    const float a = L.v.albedo;
+    const float a = L.v->albedo;
     if (!((a > 0.f) && ((a >= 1.f) || (draw_u(L) < a)))) {
         photon_end(L);
         return;
     }
-    const float dist = -L.v.mfp * logf(draw_u(L));
+    const float dist = -L.v->mfp * logf(draw_u(L));
     L.c.total_dist = dist;
     L.c.first_flight = 1;
     flight_start(L, dist);
@@ -591,11 +597,11 @@ __device__ void direct_done(Lane &L, int res) {
         if (exit_to_sensor(S, p, h.v, h.n0, Lv, fore, fw, dts)) {
             h.L = Lv;
             count(L, C_DIRECT);
-            const float val = L.v.weight * expf(-L.v.sigma_t * c.acc) * fw;
+            const float val = L.v->weight * expf(-L.v->sigma_t * c.acc) * fw;
             if constexpr (MODE == M_ADJ) {
                 if (direct_adj_start<FIELD>(L, p, Lv, val)) return;
             } else {
-                splat(S, L.A, L.v.image, p, Lv, 0, val);
+                splat(S, L.A, L.v->image, p, Lv, 0, val);
             }
         }
     }
@@ -613,12 +619,12 @@ __device__ void nee_done(Lane &L, int res) {
         if (exit_to_sensor(S, p, h.v, h.n0, Lv, fore, fw, dts)) {
             const f3 din = c.mv / len(c.mv);
             const float val =
-                L.v.weight * (2.f * kPi) * expf(-L.v.sigma_t * c.acc) * hg_f(L.v.g, din, dts) * fore * fw;
+                L.v->weight * (2.f * kPi) * expf(-L.v->sigma_t * c.acc) * hg_f(L.v->g, din, dts) * fore * fw;
             count(L, C_NEE);
             if constexpr (MODE == M_ADJ) {
                 if (nee_adj_start<FIELD>(L, p, Lv, val, din, dts, fore)) return;
             } else {
-                splat(S, L.A, L.v.image, p, Lv, c.depth, val);
+                splat(S, L.A, L.v->image, p, Lv, c.depth, val);
             }
         }
     }
@@ -756,8 +762,8 @@ __device__ void photon_init(Lane &L) {
         }
         return;
     }
-    h.tau = L.v.phi_min + L.v.phi_range * u[5];
-    h.scaling = fmaxf(fminf(sinf(h.tau), L.v.n_max_scaling), -L.v.n_max_scaling);
+    h.tau = L.v->phi_min + L.v->phi_range * u[5];
+    h.scaling = fmaxf(fminf(sinf(h.tau), L.v->n_max_scaling), -L.v->n_max_scaling);
     h.p = pos;
     refresh_field<FIELD>(L);
     h.v = h.n0 * dir;
@@ -807,21 +813,25 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FIE
 k_render(DevScene S, RenderArgs A) {
     __shared__ Cold cold[kBlock];
     __shared__ uint32_t cnt[C_N];
-    __shared__ Var vars[BATCH ? kBlock : 1];
-    __shared__ float4 jlds[JLDS ? 4 * kJtabLds : 1];
+    __shared__ float4 jlds[JLDS ? (BATCH ? kJtabLdsBatch : 4 * kJtabLds) : 1];
     __shared__ float4 mcache[kBlock];
     if (threadIdx.x < C_N) cnt[threadIdx.x] = 0u;
-    if (JLDS)
-        for (int i = threadIdx.x; i < 4 * S.jtab_n; i += blockDim.x) jlds[i] = S.jtab[i];
+    if (JLDS) {
+        const float4 *src = BATCH ? A.jtab_base : S.jtab;
+        const int nj = BATCH ? A.jtab_len : 4 * S.jtab_n;
+        for (int i = threadIdx.x; i < nj; i += blockDim.x) jlds[i] = src[i];
+    }
     __syncthreads();
     Var vsingle = var_from(S, A);
-    if (JLDS) vsingle.f.jtab = jlds;
-    Var &v = BATCH ? vars[threadIdx.x] : vsingle;
+    if (JLDS && !BATCH) vsingle.f.jtab = jlds;
+    // batch: the lane's configuration, read from global memory (L1/L2
+    // resident), replaced when the lane takes a photon of another one
+    Var *vp = BATCH ? const_cast<Var *>(A.cfgs) : &vsingle;
     Hot h;
     h.steps = 0;
     h.state = ST_NEED;
     h.dead = false;
-    Lane L{h, cold[threadIdx.x], v, S, A, cnt, nullptr, &mcache[threadIdx.x]};
+    Lane L{h, cold[threadIdx.x], vp, S, A, cnt, nullptr, &mcache[threadIdx.x]};
     const int lane = __lane_id();
     unsigned long long wave_steps = 0;  // ER steps of the wave (uniform)
     while (true) {
@@ -847,8 +857,7 @@ k_render(DevScene S, RenderArgs A) {
                                 hi = mid;
                         }
                         pid = idx - A.prefix[lo];
-                        v = A.cfgs[lo];
-                        v.image = A.image + (size_t)lo * A.X * A.Y * A.Z;
+                        L.v = const_cast<Var *>(A.cfgs + lo);  // its image pointer is set per launch
                     }
                     L.c.pid_lo = (uint32_t)pid;
                     L.c.pid_hi = (uint32_t)(pid >> 32);
@@ -862,8 +871,9 @@ k_render(DevScene S, RenderArgs A) {
         if (h.state == ST_SEG_END) seg_end<FIELD, MODE>(L);
         if (__ballot(h.state != ST_DONE) == 0ull) break;
         if (__ballot(h.state == ST_MARCH) == 0ull) continue;
-        const FieldP F = v.f;
-        const float Lcut = v.Lcut;
+        FieldP F = L.v->f;
+        if (BATCH && JLDS) F.jtab = jlds + (F.jtab - A.jtab_base);
+        const float Lcut = L.v->Lcut;
         bool m = h.state == ST_MARCH;
         uint32_t sl = h.steps_left;
         for (int k = 0; k < A.spc; ++k) {
@@ -1184,7 +1194,7 @@ static int check_desc(simpler_ctx *ctx, const simpler_scene_desc *d) {
     if (d->field_kind < 0 || d->field_kind > 2) return fail(ctx, SIMPLER_ERR_INVALID, "field_kind out of range");
     if (!(d->er_stepsize > 0.f)) return fail(ctx, SIMPLER_ERR_INVALID, "er_stepsize must be > 0");
     if (d->precision < 0 || d->precision > 12) return fail(ctx, SIMPLER_ERR_INVALID, "precision out of range [0,12]");
-    if (d->mode < 0 || d->mode > 7) return fail(ctx, SIMPLER_ERR_INVALID, "Bessel mode out of range [0,7]");
+    if (d->mode < 0 || d->mode > kMaxMode) return fail(ctx, SIMPLER_ERR_INVALID, "Bessel mode out of range [0,7]");
     if (!(d->f_u > 0.f) || !(d->speed_u > 0.f)) return fail(ctx, SIMPLER_ERR_INVALID, "f_u and speed_u must be > 0");
     if (!(d->view_plane[0] > 0.f) || !(d->view_plane[1] > 0.f)) return fail(ctx, SIMPLER_ERR_INVALID, "view_plane must be > 0");
     for (int i = 0; i < 3; ++i)
@@ -1958,6 +1968,7 @@ struct simpler_batch {
     Var *d_cfgs = nullptr;
     unsigned long long *d_prefix = nullptr;
     float4 *d_jtab = nullptr;
+    int jtab_len = 0;  // float4 in d_jtab (one table per Bessel order present)
     int field = SIMPLER_FIELD_BESSEL;
     float4 *d_src = nullptr;  // point-source patterns: n_cfg x n_src (x, y, z, amplitude)
     float *d_phase = nullptr;  // and their phases
@@ -1966,6 +1977,13 @@ struct simpler_batch {
     hipGraphExec_t exec = nullptr;
     bool counted = false;  // registered with ctx->live / base->batches
 };
+
+// Each configuration's image pointer in the batch's device parameters
+// (render launches write configuration c into images + c X Y Z).
+__global__ void k_cfg_images(Var *cfgs, int n_cfg, float *images, size_t xyz) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n_cfg) cfgs[c].image = images + (size_t)c * xyz;
+}
 
 static int batch_launch(simpler_batch *b, float *images, int accumulate, hipStream_t stream) {
     simpler_ctx *ctx = b->ctx;
@@ -1983,7 +2001,11 @@ static int batch_launch(simpler_batch *b, float *images, int accumulate, hipStre
     A.cfgs = b->d_cfgs;
     A.prefix = b->d_prefix;
     A.n_cfg = b->n_cfg;
+    A.jtab_base = b->d_jtab;
+    A.jtab_len = b->jtab_len;
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, stream, ctx->d_scratch, 16);
+    hipLaunchKernelGGL(k_cfg_images, dim3((b->n_cfg + 255) / 256), dim3(256), 0, stream, b->d_cfgs, b->n_cfg, images,
+                       (size_t)b->X * b->Y * b->Z);
     if (!accumulate) {
         const size_t npix = (size_t)b->n_cfg * b->X * b->Y * b->Z;
         const int blocks = (int)std::min<size_t>((npix + 255) / 256, 8192);
@@ -1994,6 +2016,9 @@ static int batch_launch(simpler_batch *b, float *images, int accumulate, hipStre
         if (b->field == SIMPLER_FIELD_SOURCES)
             hipLaunchKernelGGL((k_render<kSources, M_RENDER, true>), dim3(blocks), dim3(ctx->block), 0, stream, b->S,
                                A);
+        else if (b->jtab_len <= kJtabLdsBatch)
+            hipLaunchKernelGGL((k_render<kBessel, M_RENDER, true, true>), dim3(blocks), dim3(ctx->block), 0, stream,
+                               b->S, A);
         else
             hipLaunchKernelGGL((k_render<kBessel, M_RENDER, true>), dim3(blocks), dim3(ctx->block), 0, stream, b->S,
                                A);
@@ -2015,8 +2040,11 @@ int simpler_batch_create(simpler_scene *base, int n_cfg, const simpler_batch_cfg
         return fail(ctx, SIMPLER_ERR_INVALID, "batch: bad image resolution");
     std::vector<Var> vars(n_cfg);
     std::vector<unsigned long long> prefix(n_cfg + 1, 0ull);
-    std::vector<float> jtab_all;
-    std::vector<size_t> jtab_off(n_cfg);
+    // A Bessel table depends only on the order m and its length (interval i
+    // covers kr in [i, i+1]), so the configurations of one order share the
+    // longest of their tables.
+    std::vector<std::vector<float>> mode_tab(kMaxMode + 1);
+    std::vector<int> cfg_mode(n_cfg);
     for (int i = 0; i < n_cfg; ++i) {
         const simpler_batch_cfg &c = cfgs[i];
         if (c.n_photons < 0) return fail(ctx, SIMPLER_ERR_INVALID, "batch: n_photons must be >= 0");
@@ -2036,8 +2064,8 @@ int simpler_batch_create(simpler_scene *base, int n_cfg, const simpler_batch_cfg
         int nint = 0;
         rc = bessel_table_host(ctx, tmp.desc, tab, nint);
         if (rc) return rc;
-        jtab_off[i] = jtab_all.size();
-        jtab_all.insert(jtab_all.end(), tab.begin(), tab.end());
+        cfg_mode[i] = tmp.desc.mode;
+        if (tab.size() > mode_tab[tmp.desc.mode].size()) mode_tab[tmp.desc.mode].swap(tab);
         const DevScene Si = make_dev_scene(&tmp, c.n_photons > 0 ? c.n_photons : 1, c.seed, false);
         Var &v = vars[i];
         v.f.amp = Si.amp;
@@ -2058,8 +2086,14 @@ int simpler_batch_create(simpler_scene *base, int n_cfg, const simpler_batch_cfg
         v.Lcut = Si.Lcut;
         v.seed_lo = Si.seed_lo;
         v.seed_hi = Si.seed_hi;
-        v.image = nullptr;  // set in the kernel from the configuration index
+        v.image = nullptr;  // set per launch (k_cfg_images)
         prefix[i + 1] = prefix[i] + (unsigned long long)c.n_photons;
+    }
+    std::vector<float> jtab_all;
+    std::vector<size_t> mode_off(kMaxMode + 1, 0);
+    for (int m = 0; m <= kMaxMode; ++m) {
+        mode_off[m] = jtab_all.size();
+        jtab_all.insert(jtab_all.end(), mode_tab[m].begin(), mode_tab[m].end());
     }
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     simpler_batch *b = new simpler_batch();
@@ -2076,7 +2110,8 @@ int simpler_batch_create(simpler_scene *base, int n_cfg, const simpler_batch_cfg
         simpler_batch_destroy(b);
         return fail(ctx, SIMPLER_ERR_NOMEM, "batch: device allocation failed");
     }
-    for (int i = 0; i < n_cfg; ++i) vars[i].f.jtab = b->d_jtab + jtab_off[i] / 4;
+    b->jtab_len = (int)(jtab_all.size() / 4);
+    for (int i = 0; i < n_cfg; ++i) vars[i].f.jtab = b->d_jtab + mode_off[cfg_mode[i]] / 4;
     if (hipMemcpy(b->d_jtab, jtab_all.data(), jtab_all.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_cfgs, vars.data(), vars.size() * sizeof(Var), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_prefix, prefix.data(), prefix.size() * sizeof(unsigned long long), hipMemcpyHostToDevice) !=
