@@ -31,10 +31,10 @@ F_STEP = {0: 92, 1: 126, 2: 148}
 F_EVENT = dict(scatters=80, nee_splats=127, direct_splats=100, face_events=45, photons=40)
 FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per photon of k_render (mode 0), from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE of one 2^20-photon launch (3,938.3 and 1,307,812.0 KiB),
+# and WRITE_SIZE of one 2^20-photon launch (6,489.6 and 1,311,486.8 KiB),
 # scaled by the counters' reading for a 1 GiB device copy (x2.00 fetch,
 # x1.00 write): almost all of it is the splat atomics.
-TRAFFIC_BYTES_PER_PHOTON = 1284.85
+TRAFFIC_BYTES_PER_PHOTON = 1293.42
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per sample of the three prefilter passes on a 512^3 grid
 # (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of one call, same scaling):
