@@ -10,7 +10,7 @@ import math
 
 import numpy as np
 
-from ._abi import (FIELD_BESSEL, FIELD_SOURCES, FIELD_SPLINE, FLAGS_DEFAULT,
+from ._abi import (FIELD_BESSEL, FIELD_SOURCES, FIELD_SPLINE, FLAG_GENERAL_AXIS, FLAGS_DEFAULT,
                    SceneDesc)
 
 
@@ -189,6 +189,21 @@ def fus_sources_scene(res=(1024, 1024, 1)):
     src = reference_arc_sources(0.0508, (0.0, 0.0, -0.0508), (0.0, -0.0508, 0.0), True, True, math.pi, math.pi,
                                 -th, th, 100, -0.0127 / 2, 0.0127 / 2, 51, 2.15e-5, 1)
     return d, disk_texture(101), res, src
+
+
+def vol_bessel_desc(general_axis=True):
+    """The Bessel field of renderings/testing_splines.sh (f_u 898.1 kHz,
+    1500 m/s, n_o 1.3333, n_max 0.0278, n_clip 1, phi = pi/2, m = 0, axis
+    -x through the origin) in its 30 mm slab (x in +-15 mm): the field that
+    the reference's renderings/besselriffile.vol samples on a 101^3 grid."""
+    d = base_desc(half=0.015, step=1e-3, precision=4)
+    d.set(block_l=(-0.015, -5.0, -5.0), block_r=(0.015, 5.0, 5.0), egap_end_loc_x=0.015,
+          sgap_begin_loc_x=-0.015, f_u=898100.0, speed_u=1500.0, n_o=1.3333, n_max=0.0278, n_clip=1.0,
+          phi_min=math.pi / 2, phi_max=math.pi / 2, mode=0, axis_uz=(-1.0, 0.0, 0.0), axis_ux=(0.0, 0.0, 1.0),
+          p_u=(0.0, 0.0, 0.0))
+    if not general_axis:
+        d.set(flags=d.flags & ~FLAG_GENERAL_AXIS)
+    return d
 
 
 def spline_cube(n=64, sigma_s=2000.0, g=0.9, step=5e-6, res=(64, 64, 32), seed=0):

@@ -13,6 +13,9 @@ Sources (data only):
   splinepractice/interpolate3d.csv   7 derivative orders at 576 points
                                      (inputs: spline_tests.cpp:100-134)
   renderer/images/Disk_501.pfm       inscribed-disk emitter texture
+  renderings/besselriffile.vol       the 101^3 grid of n that renderings/testing_splines.sh
+                                     renders (rifgridFile); it does not vary along x, so one
+                                     y-z slice and the header reproduce it exactly
 """
 import json
 import os
@@ -72,7 +75,17 @@ def main():
     rows = [[int(np.argmax(r)), int(len(r) - np.argmax(r[::-1]))] if r.any() else [0, 0] for r in lit]
     json.dump({"shape": [501, 501], "lit": int(lit.sum()), "values": sorted(set(np.unique(px).tolist())),
                "row_ranges": rows}, open(os.path.join(HERE, "disk_501.json"), "w"))
-    print("wrote", sorted(out), "and disk_501.json")
+    # VOL grid: header, x-invariance check, the y-z slice at x index 50
+    raw = open(os.path.join(REF, "renderings/besselriffile.vol"), "rb").read()
+    assert raw[:3] == b"VOL" and raw[3] == 3
+    N = np.frombuffer(raw[8:20], "<i4")
+    lo = np.frombuffer(raw[24:36], "<f4")
+    hi = np.frombuffer(raw[36:48], "<f4")
+    grid = np.frombuffer(raw[48:], "<f4").reshape(N[2], N[1], N[0])
+    xvar = float(np.abs(grid - grid[:, :, :1]).max())
+    np.savez_compressed(os.path.join(HERE, "besselriffile_slice.npz"), N=N, xmin=lo, xmax=hi,
+                        slice_zy=grid[:, :, N[0] // 2].copy(), x_variation=np.float64(xvar))
+    print("wrote", sorted(out), "disk_501.json and besselriffile_slice.npz")
 
 
 if __name__ == "__main__":

@@ -124,3 +124,37 @@ def test_non_finite_sample_is_rejected(gpu_ctx):
         sc.set_spline_samples(g, (4, 5, 6), (-1, -1, -1), (1, 1, 1))
     with pytest.raises(engine.SimplerError):
         sc.set_spline_samples(np.ones(8, np.float32), (1, 2, 4), (-1, -1, -1), (1, 1, 1))
+
+
+def test_reference_vol_grid_as_spline_matches_the_bessel_field(gpu_ctx):
+    """renderings/besselriffile.vol (101^3 samples of the testing_splines.sh
+    Bessel field; constant along x, so the committed y-z slice rebuilds it
+    exactly): the GPU Bessel field reproduces its nodes, and the spline scene
+    built from the grid (prefiltered on the GPU) matches the Bessel field
+    between the nodes -- the comparison of renderer/samples/testUS.cpp."""
+    g = np.load(os.path.join(G, "besselriffile_slice.npz"))
+    N, lo, hi, sl = g["N"], g["xmin"], g["xmax"], g["slice_zy"]
+    assert float(g["x_variation"]) == 0.0
+    grid = np.repeat(sl[:, :, None], int(N[0]), axis=2)  # (z, y, x), x fastest
+    d = scenes.vol_bessel_desc(True)
+    tex = scenes.disk_texture(11)
+    bes = engine.Scene(gpu_ctx, d, tex)
+    ys = np.linspace(lo[1], hi[1], N[1])
+    zs = np.linspace(lo[2], hi[2], N[2])
+    zz, yy = np.meshgrid(zs, ys, indexing="ij")
+    nodes = np.stack([np.zeros(zz.size), yy.ravel(), zz.ravel()], axis=1)
+    n_nodes, _ = bes.field(torch.tensor(nodes, dtype=torch.float32, device="cuda"))
+    assert np.abs(n_nodes.double().cpu().numpy().reshape(sl.shape) - sl).max() < 2.5e-7
+    ds = d.copy()
+    ds.set(field_kind=2)
+    spl = engine.Scene(gpu_ctx, ds, tex)
+    spl.set_spline_samples(torch.tensor(grid.reshape(-1), device="cuda"), [int(x) for x in N], list(lo), list(hi))
+    rng = np.random.default_rng(8)
+    pts = np.stack([rng.uniform(-0.014, 0.014, 4096), rng.uniform(-0.00095, 0.00095, 4096),
+                    rng.uniform(-0.00095, 0.00095, 4096)], axis=1)
+    tp = torch.tensor(pts, dtype=torch.float32, device="cuda")
+    (n_s, g_s), (n_b, g_b) = spl.field(tp), bes.field(tp)
+    en = (n_s - n_b).abs().max().item()
+    eg = ((g_s - g_b).abs().max() / g_b.abs().max()).item()
+    print(f"spline vs Bessel between nodes: n {en:.2e}, grad {eg:.2e} of max", flush=True)
+    assert en < 3e-7 and eg < 2e-3
