@@ -149,12 +149,24 @@ def test_reference_vol_grid_as_spline_matches_the_bessel_field(gpu_ctx):
     ds.set(field_kind=2)
     spl = engine.Scene(gpu_ctx, ds, tex)
     spl.set_spline_samples(torch.tensor(grid.reshape(-1), device="cuda"), [int(x) for x in N], list(lo), list(hi))
+    # points away from the grid's y / z faces: within a few cells of a face
+    # the mirror boundary (zero slope there) departs from the field, in the
+    # FP64 oracle spline as well (5e-6 at 2.5 cells)
     rng = np.random.default_rng(8)
-    pts = np.stack([rng.uniform(-0.014, 0.014, 4096), rng.uniform(-0.00095, 0.00095, 4096),
-                    rng.uniform(-0.00095, 0.00095, 4096)], axis=1)
+    pts = np.stack([rng.uniform(-0.014, 0.014, 4096), rng.uniform(-0.0008, 0.0008, 4096),
+                    rng.uniform(-0.0008, 0.0008, 4096)], axis=1).astype(np.float32)
     tp = torch.tensor(pts, dtype=torch.float32, device="cuda")
     (n_s, g_s), (n_b, g_b) = spl.field(tp), bes.field(tp)
     en = (n_s - n_b).abs().max().item()
     eg = ((g_s - g_b).abs().max() / g_b.abs().max()).item()
     print(f"spline vs Bessel between nodes: n {en:.2e}, grad {eg:.2e} of max", flush=True)
-    assert en < 3e-7 and eg < 2e-3
+    assert en < 3e-7 and eg < 1e-3
+    # and against the FP64 oracle spline of the same grid at the same points
+    Ni = [int(x) for x in N]
+    c64 = O.spline_build3d(grid.reshape(-1).astype(np.float64), Ni)
+    p64 = pts.astype(np.float64)
+    n64 = O.spline_eval3d(c64, Ni, list(lo.astype(float)), list(hi.astype(float)), 0, 0, 0, p64)
+    g64 = np.stack([O.spline_eval3d(c64, Ni, list(lo.astype(float)), list(hi.astype(float)), *o, p64)
+                    for o in ((1, 0, 0), (0, 1, 0), (0, 0, 1))], axis=1)
+    assert np.abs(n_s.double().cpu().numpy() - n64).max() < 2e-7
+    assert np.abs(g_s.double().cpu().numpy() - g64).max() < 1e-4 * np.abs(g64).max()
