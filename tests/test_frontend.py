@@ -293,3 +293,32 @@ def test_renderer3d_sample_defaults_and_errors(tmp_path, texture):
                      (["emitter_gap=1.0"], "invalid gap")):
         r, _ = run_sample(tmp_path, [f"projectorTexture={texture}"] + bad)
         assert r.returncode == 1 and msg in r.stderr, (bad, r.stderr)
+
+
+def test_json_reader_escapes_numbers_and_malformed_documents(tmp_path):
+    """The settings reader: \\u and other escapes in strings (a texture path
+    with a space and slashes written as \\u escapes), exponents and integral
+    floats where integers are read, a repeated key (the last one counts), and
+    malformed documents rejected with the position."""
+    sub = tmp_path / "tex dir"
+    sub.mkdir()
+    tex = sub / "d.pfm"
+    write_pfm_double(str(tex), scenes.disk_texture(11))
+    escaped = str(tex).replace("/", "\\u002F").replace(" ", "\\u0020")
+    s = copy.deepcopy(FUS)
+    s["projector_texture"] = "PLACEHOLDER"
+    s["film_parameters"] = {"path_length": {"min": 0.0, "max": 1e5, "bins": 2.0}, "spatial": {"x": 32, "y": 16}}
+    text = json.dumps(s).replace('"PLACEHOLDER"', '"' + escaped + '"')
+    # a repeated key: the later value is the one read
+    text = text.replace('"threads": -1', '"threads": 3, "threads": -1')
+    r, d = run_simpler(tmp_path, text)
+    assert r.returncode == 0, r.stderr
+    assert (d["tex_x"], d["tex_y"]) == (11, 11)
+    assert d["pathlength_range"] == [0, 100000] and (d["x_res"], d["y_res"], d["z_res"]) == (32, 16, 2)
+    assert d["threads"] == -1
+    for bad, what in (('{"stricts": true,}', "expected a key string"), ('{"stricts": tru}', "unexpected character"),
+                      ('{"a": "no end}', "unterminated string"), ('{"a": 1} x', "trailing characters"),
+                      ('{"a": 01}', "expected ',' or '}'"), ('{"a": "\\q"}', "bad escape"),
+                      ('{"a": "\\ud800"}', "lone high surrogate"), ('{"a": -}', "bad number")):
+        r, _ = run_simpler(tmp_path, bad)
+        assert r.returncode == 1 and "JSON parse error at line 1" in r.stderr and what in r.stderr, (bad, r.stderr)
