@@ -107,7 +107,7 @@ __device__ __forceinline__ void rec_add_lambda(float *base, int idx, f3 lamp, f3
 
 __device__ __forceinline__ float level_h(float h0, int lev) {
     float h = h0;
-    for (int i = 0; i < lev; ++i) h = h / 10.0f;  // the forward refinement chain
+    for (int i = 0; i < lev; ++i) h = div10(h);  // the forward refinement chain
     return h;
 }
 

@@ -59,6 +59,16 @@ __device__ __forceinline__ float to_uniform(uint32_t w) {
     return (float)(((w >> 9) << 1) | 1u) * (1.0f / 16777216.0f);
 }
 
+// x / 10 correctly rounded without the IEEE division sequence: q0 = x *
+// RN(0.1), the exact residual x - 10 q0 by one fma, one correction step.
+// Equal to the FP32 quotient x / 10.0f for every |x| >= 1e-30 (checked over
+// all such floats); below that, in the subnormal range, it can differ by an
+// ulp.
+__device__ __forceinline__ float div10(float x) {
+    const float q0 = x * 0.1f;
+    return __builtin_fmaf(__builtin_fmaf(-q0, 10.0f, x), 0.1f, q0);
+}
+
 // Two-sum accumulation hi + lo += d (Kahan's compensated step).
 __device__ __forceinline__ void comp_add(float &hi, float &lo, float d) {
     const float y = d + lo;

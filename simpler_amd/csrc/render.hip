@@ -276,7 +276,7 @@ __device__ __forceinline__ bool march_step(Hot &h, uint32_t &steps_left, const D
         return true;
     }
     if (reject) {  // restore and refine: h/10, 10 more steps (scene.cpp:408-412)
-        h.h = hs / 10.0f;
+        h.h = div10(hs);
         steps_left = 10;
         return false;
     }
@@ -873,7 +873,11 @@ k_render(DevScene S, RenderArgs A) {
         if (__ballot(h.state == ST_MARCH) == 0ull) continue;
         FieldP F = L.v->f;
         if (BATCH && JLDS) F.jtab = jlds + (F.jtab - A.jtab_base);
-        const float Lcut = L.v->Lcut;
+        float Lcut = L.v->Lcut;
+        // held in a vector register through the march loop: the kernel's
+        // scalar registers are full, and a scalar copy is reloaded from its
+        // spill lanes on every step
+        asm volatile("" : "+v"(Lcut));
         bool m = h.state == ST_MARCH;
         uint32_t sl = h.steps_left;
         for (int k = 0; k < A.spc; ++k) {
