@@ -137,8 +137,12 @@ __device__ __forceinline__ void bessel_eval(const DevScene &S, const FieldP &F, 
     if (MM != 0) {
         const float c1 = nz ? dot(rho, S.ux) * ir : 1.0f;
         const float s1 = nz ? dot(rho, S.uy) * ir : 0.0f;
+        if (MM > 0) {  // the first product with (1, 0) is (c1, s1)
+            cm = c1;
+            sm = s1;
+        }
 #pragma unroll
-        for (int k = 0; k < (MM >= 0 ? MM : mode); ++k) {
+        for (int k = MM > 0 ? 1 : 0; k < (MM >= 0 ? MM : mode); ++k) {
             const float cn = cm * c1 - sm * s1;
             sm = sm * c1 + cm * s1;
             cm = cn;
