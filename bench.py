@@ -31,10 +31,10 @@ F_STEP = {0: 92, 1: 126, 2: 148}
 F_EVENT = dict(scatters=80, nee_splats=127, direct_splats=100, face_events=45, photons=40)
 FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per photon of k_render (mode 0), from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE of one 2^20-photon launch (6,489.6 and 1,311,486.8 KiB),
+# and WRITE_SIZE of one 2^20-photon launch (8,410.6 and 1,316,608.4 KiB),
 # scaled by the counters' reading for a 1 GiB device copy (x2.00 fetch,
 # x1.00 write): almost all of it is the splat atomics.
-TRAFFIC_BYTES_PER_PHOTON = 1293.42
+TRAFFIC_BYTES_PER_PHOTON = 1302.18
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per sample of the three prefilter passes on a 512^3 grid
 # (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of one call, same scaling):
@@ -75,8 +75,10 @@ def gpu_bench(args):
         X, Y, Z = res
         img = torch.zeros((Z, Y, X), dtype=torch.float32, device="cuda")
         work[m] = (sc, img, res)
-    n_tune = args.photons_per_step if args.autotune_photons < 0 else args.autotune_photons
-    if not args.tune and n_tune > 0:  # the autotuner picks the launch shape on the workload itself
+    # the autotuner picks the launch shape on the workload itself, on a chunk
+    # of up to 2^24 photons (long enough that the launch drain does not decide)
+    n_tune = min(args.photons_per_step, 1 << 24) if args.autotune_photons < 0 else args.autotune_photons
+    if not args.tune and n_tune > 0:
         autotuned = work[modes[0]][0].autotune(n_tune, work[modes[0]][2])
         shape = ctxs[modes[0]].get_tuning()
         for c in ctxs.values():
@@ -442,11 +444,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--photons-per-step", type=int, default=1 << 23)
+    ap.add_argument("--photons-per-step", type=int, default=1 << 25)
     ap.add_argument("--modes", default="0,2")
     ap.add_argument("--tune", default="", help="block_size,steps_per_chunk,blocks_per_cu (skips the autotuner)")
     ap.add_argument("--autotune-photons", type=int, default=-1,
-                    help="photons per autotuner trial (default: the step's own chunk size; 0: no autotuning)")
+                    help="photons per autotuner trial (default: the step's chunk size, at most 2^24; 0: no autotuning)")
     ap.add_argument("--cpu-photons", type=int, default=96)
     ap.add_argument("--cpu-photons-cutoff", type=int, default=768)
     ap.add_argument("--cpu-threads", type=int, default=0)
