@@ -388,7 +388,7 @@ int simpler_ctx_get_tuning(simpler_ctx *ctx, int *block_size,
 /* Autotuner (no reference counterpart: the reference hard-codes its launch,
  * cuda_renderer.cu:763-767): times an n_photons render of the scene at the
  * given resolution for every candidate (block size 128/256, steps per chunk
- * 16/32/64, blocks per CU 3/4/5), keeps the fastest on the scene's context
+ * 32/64/128, blocks per CU 3/4/5), keeps the fastest on the scene's context
  * and returns it (outputs may be NULL).  The splat has a single strategy
  * (per-tap device atomics), so it is not a candidate.  Synchronous. */
 int simpler_autotune(simpler_scene *scene, int64_t n_photons, int x_res,

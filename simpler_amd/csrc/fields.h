@@ -113,7 +113,11 @@ __device__ __forceinline__ void field_eval(const DevScene &S, const FieldP &F, f
 //                                           - J m sin(m phi) (uz x rho)/r^2]
 // MM >= 0: the order m known at compile time (m = 0 drops the angular
 // terms; the angle-addition recurrence unrolls); MM < 0: m = F.mode.
-// GA >= 0: the axis form known at compile time; GA < 0: S.general_axis.
+// GA >= 0: the axis form known at compile time (0: the reference's
+// component form, 1: general axis, 2: general axis with the frame
+// uz = +z, ux = +x, uy = +y, where the projection onto the plane normal to
+// the axis is rho = (q.x, q.y, 0) and uz x rho = (-rho.y, rho.x, 0));
+// GA < 0: S.general_axis.
 template <int MM, int GA = -1>
 __device__ __forceinline__ void bessel_eval(const DevScene &S, const FieldP &F, f3 p, float scaling, float &n,
                                             f3 &g) {
@@ -124,10 +128,17 @@ __device__ __forceinline__ void bessel_eval(const DevScene &S, const FieldP &F, 
     // costs one select instead of a branch around the evaluation
     const bool in_gap = !(p.x > S.egap || p.x < S.sgap);
     const float A = in_gap ? F.amp * scaling : 0.f;
-    const f3 q = p - S.pu;
-    const float t = dot(q, S.uz);
-    f3 rho = q - t * S.uz;
-    const float r2 = dot(rho, rho);
+    f3 rho;
+    float r2;
+    if (GA == 2) {
+        rho = mk3(p.x - S.pu.x, p.y - S.pu.y, 0.f);
+        r2 = rho.x * rho.x + rho.y * rho.y;
+    } else {
+        const f3 q = p - S.pu;
+        const float t = dot(q, S.uz);
+        rho = q - t * S.uz;
+        r2 = dot(rho, rho);
+    }
     const bool nz = r2 > 0.f;
     float ir = nz ? __builtin_amdgcn_rsqf(r2) : 0.f;
     float r = r2 * ir;
@@ -135,8 +146,8 @@ __device__ __forceinline__ void bessel_eval(const DevScene &S, const FieldP &F, 
     // angle-addition recurrence (atan2(0,0) = 0 at r = 0)
     float cm = 1.0f, sm = 0.0f;
     if (MM != 0) {
-        const float c1 = nz ? dot(rho, S.ux) * ir : 1.0f;
-        const float s1 = nz ? dot(rho, S.uy) * ir : 0.0f;
+        const float c1 = nz ? (GA == 2 ? rho.x : dot(rho, S.ux)) * ir : 1.0f;
+        const float s1 = nz ? (GA == 2 ? rho.y : dot(rho, S.uy)) * ir : 0.0f;
         if (MM > 0) {  // the first product with (1, 0) is (c1, s1)
             cm = c1;
             sm = s1;
@@ -155,7 +166,9 @@ __device__ __forceinline__ void bessel_eval(const DevScene &S, const FieldP &F, 
         // (reference scene.cpp:209-214 vs 229-234)
         bessel_tab(F, F.k_r * r, J, dJ);
         n = S.n_o + A * J * cm;
-        if (general) {
+        if (GA == 2) {
+            rho = mk3(kEps, kEps, 0.f);
+        } else if (general) {
             rho = kEps * (S.ux + S.uy);
         } else {
             rho.y = kEps;
@@ -168,11 +181,16 @@ __device__ __forceinline__ void bessel_eval(const DevScene &S, const FieldP &F, 
     if (!clamp) n = S.n_o + A * J * cm;
     const float cr = (A * F.k_r) * dJ * ir * cm;
     if (MM == 0) {
-        g = general ? cr * rho : mk3(0.f, cr * rho.y, cr * rho.z);
+        if (GA == 2)
+            g = mk3(cr * rho.x, cr * rho.y, 0.f);
+        else
+            g = general ? cr * rho : mk3(0.f, cr * rho.y, cr * rho.z);
         return;
     }
     const float ctan = A * J * (float)mode * sm * ir * ir;
-    if (general) {
+    if (GA == 2) {
+        g = mk3(cr * rho.x + ctan * rho.y, cr * rho.y - ctan * rho.x, 0.f);
+    } else if (general) {
         const f3 tp = cross(S.uz, rho);
         g = mk3(cr * rho.x - ctan * tp.x, cr * rho.y - ctan * tp.y, cr * rho.z - ctan * tp.z);
     } else {

@@ -917,6 +917,21 @@ __global__ void k_field_eval(DevScene S, int n, const float *pts, float scaling,
     out_g[3 * i + 2] = g.z;
 }
 
+// the Bessel evaluation of the single-scene render kernels, specialised the
+// same way (order, axis form), for the field-evaluation entry point
+template <int MM, int GA>
+__global__ void k_bessel_field_eval(DevScene S, int n, const float *pts, float scaling, float *out_n, float *out_g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float nn;
+    f3 g;
+    bessel_eval<MM, GA>(S, field_params(S), mk3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]), scaling, nn, g);
+    out_n[i] = nn;
+    out_g[3 * i] = g.x;
+    out_g[3 * i + 1] = g.y;
+    out_g[3 * i + 2] = g.z;
+}
+
 }  // namespace simpler
 
 // ======================================================================
@@ -1206,21 +1221,38 @@ static int check_desc(simpler_ctx *ctx, const simpler_scene_desc *d) {
     return SIMPLER_OK;
 }
 
+// The specialisation of the Bessel evaluation a scene gets in single-scene
+// renders: 3 m + axis form for m = 0 or 2 (form 0: the reference's component
+// form, 1: general axis, 2: general axis in the coordinate frame uz = +z,
+// ux = +x, where the projection and the cross product reduce to components);
+// -1: the runtime order and form.
+static int bessel_variant(const DevScene &S) {
+    if (S.mode != 0 && S.mode != 2) return -1;
+    const bool zframe = S.general_axis && S.uz.x == 0.f && S.uz.y == 0.f && S.uz.z == 1.f && S.ux.x == 1.f &&
+                        S.ux.y == 0.f && S.ux.z == 0.f;
+    return 3 * S.mode + (zframe ? 2 : (S.general_axis ? 1 : 0));
+}
+
 template <int MODE>
 static void launch_render(simpler_ctx *ctx, int field, const DevScene &S, const RenderArgs &A) {
     const int blocks = std::max(1, ctx->num_cu * ctx->bpc);
     const dim3 grid(blocks), block(ctx->block);
     switch (field) {
         case SIMPLER_FIELD_BESSEL:
-            if (MODE == M_RENDER && S.jtab_n <= kJtabLds && (S.mode == 0 || S.mode == 2)) {
-                if (S.mode == 0 && S.general_axis)
-                    hipLaunchKernelGGL((k_render<kBessel, M_RENDER, false, true, 0, 1>), grid, block, 0, ctx->cur, S, A);
-                else if (S.mode == 0)
-                    hipLaunchKernelGGL((k_render<kBessel, M_RENDER, false, true, 0, 0>), grid, block, 0, ctx->cur, S, A);
-                else if (S.general_axis)
-                    hipLaunchKernelGGL((k_render<kBessel, M_RENDER, false, true, 2, 1>), grid, block, 0, ctx->cur, S, A);
-                else
-                    hipLaunchKernelGGL((k_render<kBessel, M_RENDER, false, true, 2, 0>), grid, block, 0, ctx->cur, S, A);
+            if (MODE == M_RENDER && S.jtab_n <= kJtabLds && bessel_variant(S) >= 0) {
+                switch (bessel_variant(S)) {
+#define SIMPLER_RENDER_BV(MM, GA)                                                                                  \
+    case 3 * (MM) + (GA):                                                                                          \
+        hipLaunchKernelGGL((k_render<kBessel, M_RENDER, false, true, MM, GA>), grid, block, 0, ctx->cur, S, A); \
+        break;
+                    SIMPLER_RENDER_BV(0, 0)
+                    SIMPLER_RENDER_BV(0, 1)
+                    SIMPLER_RENDER_BV(0, 2)
+                    SIMPLER_RENDER_BV(2, 0)
+                    SIMPLER_RENDER_BV(2, 1)
+                    SIMPLER_RENDER_BV(2, 2)
+#undef SIMPLER_RENDER_BV
+                }
             } else if (S.jtab_n <= kJtabLds)
                 hipLaunchKernelGGL((k_render<kBessel, MODE, false, true>), grid, block, 0, ctx->cur, S, A);
             else
@@ -1386,7 +1418,7 @@ int simpler_autotune(simpler_scene *sc, int64_t n_photons, int X, int Y, int Z, 
         (void)hipFree(img);
         return fail(ctx, SIMPLER_ERR_HIP, "autotune: event creation failed");
     }
-    const int blocks[] = {128, 256}, spcs[] = {16, 32, 64}, bpcs[] = {3, 4, 5};
+    const int blocks[] = {128, 256}, spcs[] = {32, 64, 128}, bpcs[] = {3, 4, 5};
     int best[3] = {ctx->block, ctx->spc, ctx->bpc};
     float best_ms = INFINITY;
     rc = SIMPLER_OK;
@@ -1812,7 +1844,23 @@ int simpler_field_eval(simpler_scene *sc, int n_pts, const float *pts_dev, float
     const dim3 grid((n_pts + 255) / 256), block(256);
     switch (sc->desc.field_kind) {
         case SIMPLER_FIELD_BESSEL:
-            hipLaunchKernelGGL(k_field_eval<kBessel>, grid, block, 0, ctx->cur, S, n_pts, pts_dev, scaling, tau, out_n_dev, out_g_dev);
+            switch (bessel_variant(S)) {
+#define SIMPLER_FIELD_BV(MM, GA)                                                                                  \
+    case 3 * (MM) + (GA):                                                                                         \
+        hipLaunchKernelGGL((k_bessel_field_eval<MM, GA>), grid, block, 0, ctx->cur, S, n_pts, pts_dev, scaling, \
+                           out_n_dev, out_g_dev);                                                                 \
+        break;
+                SIMPLER_FIELD_BV(0, 0)
+                SIMPLER_FIELD_BV(0, 1)
+                SIMPLER_FIELD_BV(0, 2)
+                SIMPLER_FIELD_BV(2, 0)
+                SIMPLER_FIELD_BV(2, 1)
+                SIMPLER_FIELD_BV(2, 2)
+#undef SIMPLER_FIELD_BV
+                default:
+                    hipLaunchKernelGGL(k_field_eval<kBessel>, grid, block, 0, ctx->cur, S, n_pts, pts_dev, scaling, tau,
+                                       out_n_dev, out_g_dev);
+            }
             break;
         case SIMPLER_FIELD_SOURCES:
             hipLaunchKernelGGL(k_field_eval<kSources>, grid, block, 0, ctx->cur, S, n_pts, pts_dev, scaling, tau, out_n_dev, out_g_dev);

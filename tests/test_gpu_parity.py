@@ -167,6 +167,34 @@ def _image_parity(gpu_ctx, d, tex, res, seed=3, what="image"):
     assert abs(ng - no) <= 0.002 * no + 2
 
 
+@pytest.mark.parametrize("mode", [0, 2])
+def test_coordinate_frame_kernel_equals_general_axis_kernel(gpu_ctx, mode):
+    """Single-scene renders specialise the Bessel evaluation for the axis
+    frame uz = +z, ux = +x (rho = (q.x, q.y, 0)).  Against the general-axis
+    kernel on the same field written in the frame ux = -x, uy = -y (for even
+    m the same field: 2 phi shifts by 2 pi): field values to an ulp, and with
+    the same photons the same images and step counts up to float rounding of
+    the field."""
+    d, tex, _ = scenes.bessel_cube(step=5e-6, n_max=2e-5, mode=mode)
+    d2 = d.copy()
+    d2.set(axis_ux=(-1.0, 0.0, 0.0))
+    sc1, sc2 = engine.Scene(gpu_ctx, d, tex), engine.Scene(gpu_ctx, d2, tex)
+    pts = torch.tensor(_pts(seed=7), dtype=torch.float32, device="cuda")
+    (n1, g1), (n2, g2) = sc1.field(pts, 0.7), sc2.field(pts, 0.7)
+    assert (n1 - n2).abs().max().item() < 2e-7
+    assert (g1 - g2).abs().max().item() < 1e-5 * g1.abs().max().item()
+    res = (32, 32, 16)
+    (a, s1), (b, s2) = sc1.render(1 << 15, res, seed=3), sc2.render(1 << 15, res, seed=3)
+    a, b = a.double().cpu().numpy(), b.double().cpu().numpy()
+    log(f"frame kernel m={mode}: L1 {np.abs(a - b).sum() / a.sum():.2e}, steps {s1['er_steps']} {s2['er_steps']}")
+    assert a.sum() > 0
+    assert abs(a.sum() - b.sum()) <= 1e-4 * a.sum()
+    assert np.abs(a - b).sum() <= 2e-3 * a.sum()
+    assert abs(s1["er_steps"] - s2["er_steps"]) <= 1e-4 * s2["er_steps"]
+    assert s1["photons"] == s2["photons"]
+    assert abs(s1["scatters"] - s2["scatters"]) <= 1e-3 * s2["scatters"] + 2
+
+
 def test_render_every_bin_within_3_sigma_of_oracle(gpu_ctx):
     d, tex, res = scenes.bessel_cube_small()
     _image_parity(gpu_ctx, d, tex, res)
