@@ -1226,11 +1226,14 @@ static int check_desc(simpler_ctx *ctx, const simpler_scene_desc *d) {
 // form, 1: general axis, 2: general axis in the coordinate frame uz = +z,
 // ux = +x, where the projection and the cross product reduce to components);
 // -1: the runtime order and form.
+static bool axis_zframe(const DevScene &S) {
+    return S.general_axis && S.uz.x == 0.f && S.uz.y == 0.f && S.uz.z == 1.f && S.ux.x == 1.f && S.ux.y == 0.f &&
+           S.ux.z == 0.f;
+}
+
 static int bessel_variant(const DevScene &S) {
     if (S.mode != 0 && S.mode != 2) return -1;
-    const bool zframe = S.general_axis && S.uz.x == 0.f && S.uz.y == 0.f && S.uz.z == 1.f && S.ux.x == 1.f &&
-                        S.ux.y == 0.f && S.ux.z == 0.f;
-    return 3 * S.mode + (zframe ? 2 : (S.general_axis ? 1 : 0));
+    return 3 * S.mode + (axis_zframe(S) ? 2 : (S.general_axis ? 1 : 0));
 }
 
 template <int MODE>
@@ -2068,6 +2071,9 @@ static int batch_launch(simpler_batch *b, float *images, int accumulate, hipStre
         if (b->field == SIMPLER_FIELD_SOURCES)
             hipLaunchKernelGGL((k_render<kSources, M_RENDER, true>), dim3(blocks), dim3(ctx->block), 0, stream, b->S,
                                A);
+        else if (b->jtab_len <= kJtabLdsBatch && axis_zframe(b->S))  // the geometry's axis frame is shared
+            hipLaunchKernelGGL((k_render<kBessel, M_RENDER, true, true, -1, 2>), dim3(blocks), dim3(ctx->block), 0,
+                               stream, b->S, A);
         else if (b->jtab_len <= kJtabLdsBatch)
             hipLaunchKernelGGL((k_render<kBessel, M_RENDER, true, true>), dim3(blocks), dim3(ctx->block), 0, stream,
                                b->S, A);
