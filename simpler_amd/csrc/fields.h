@@ -50,8 +50,9 @@ struct DevScene {
     const float *src_phase;
     float coeff_w[8];
     const float *spline;  // coefficients minus spline_ref
+    const float *spline_pad;  // the same with a mirrored halo (see kSplineHalo), or null
     float spline_ref;
-    int sN[3];
+    int sN[3], sP[3];  // grid and padded grid dimensions
     float sxmin[3], sxres[3];
     // medium
     float sigma_t, mfp, albedo, g;
@@ -268,6 +269,23 @@ __device__ __forceinline__ void bspline_w(float t, float w[4], float dw[4]) {
     dw[3] = 0.5f * t2;
 }
 
+// Mirrored halo of the padded coefficient copy: kSplineHalo cells below and
+// kSplineHalo + 1 above each axis, so that points up to a cell outside the
+// grid (a step overshooting the block) still read their taps in place.
+constexpr int kSplineHalo = 2;
+
+// One row of 4 x-taps: a0 (value weights) and a1 (x-derivative weights)
+// accumulated with the row's y weight and y-derivative weight.
+__device__ __forceinline__ void spline_row(float c0, float c1, float c2, float c3, const float wx[4],
+                                           const float dwx[4], float wyb, float dwyb, float &s0, float &sx,
+                                           float &sy0) {
+    const float a0 = c0 * wx[0] + c1 * wx[1] + c2 * wx[2] + c3 * wx[3];
+    const float a1 = c0 * dwx[0] + c1 * dwx[1] + c2 * dwx[2] + c3 * dwx[3];
+    s0 += a0 * wyb;
+    sx += a1 * wyb;
+    sy0 += a0 * dwyb;
+}
+
 template <>
 __device__ __forceinline__ void field_eval<kSpline>(const DevScene &S, const FieldP &, f3 p, float, float,
                                                     float &n, f3 &g) {
@@ -286,29 +304,46 @@ __device__ __forceinline__ void field_eval<kSpline>(const DevScene &S, const Fie
     bspline_w(gz - fz, wz, dwz);
     const int ix = (int)fx - 1, iy = (int)fy - 1, iz = (int)fz - 1;
     const int NX = S.sN[0], NY = S.sN[1], NZ = S.sN[2];
-    int xi[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) xi[a] = mirror_idx(ix + a, NX);
     float v = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
+    // taps (ix..ix+3, iy.., iz..) inside the padded grid: fixed strides
+    // from one corner, the halo holding the mirrored values
+    const int jx = ix + kSplineHalo, jy = iy + kSplineHalo, jz = iz + kSplineHalo;
+    if (S.spline_pad && (unsigned)jx <= (unsigned)(S.sP[0] - 4) && (unsigned)jy <= (unsigned)(S.sP[1] - 4) &&
+        (unsigned)jz <= (unsigned)(S.sP[2] - 4)) {
+        const long row = S.sP[0], slab = (long)S.sP[0] * S.sP[1];
+        const float *base = S.spline_pad + (long)jz * slab + (long)jy * row + jx;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int zi = mirror_idx(iz + c, NZ);
-        float s0 = 0.f, sx = 0.f, sy0 = 0.f;
+        for (int c = 0; c < 4; ++c) {
+            float s0 = 0.f, sx = 0.f, sy0 = 0.f;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int yi = mirror_idx(iy + b, NY);
-            const float *row = S.spline + ((long)zi * NY + yi) * NX;
-            const float c0 = row[xi[0]], c1 = row[xi[1]], c2 = row[xi[2]], c3 = row[xi[3]];
-            const float a0 = c0 * wx[0] + c1 * wx[1] + c2 * wx[2] + c3 * wx[3];
-            const float a1 = c0 * dwx[0] + c1 * dwx[1] + c2 * dwx[2] + c3 * dwx[3];
-            s0 += a0 * wy[b];
-            sx += a1 * wy[b];
-            sy0 += a0 * dwy[b];
+            for (int b = 0; b < 4; ++b) {
+                const float *r = base + c * slab + b * row;
+                spline_row(r[0], r[1], r[2], r[3], wx, dwx, wy[b], dwy[b], s0, sx, sy0);
+            }
+            v += s0 * wz[c];
+            vx += sx * wz[c];
+            vy += sy0 * wz[c];
+            vz += s0 * dwz[c];
         }
-        v += s0 * wz[c];
-        vx += sx * wz[c];
-        vy += sy0 * wz[c];
-        vz += s0 * dwz[c];
+    } else {
+        int xi[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) xi[a] = mirror_idx(ix + a, NX);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int zi = mirror_idx(iz + c, NZ);
+            float s0 = 0.f, sx = 0.f, sy0 = 0.f;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int yi = mirror_idx(iy + b, NY);
+                const float *row = S.spline + ((long)zi * NY + yi) * NX;
+                spline_row(row[xi[0]], row[xi[1]], row[xi[2]], row[xi[3]], wx, dwx, wy[b], dwy[b], s0, sx, sy0);
+            }
+            v += s0 * wz[c];
+            vx += sx * wz[c];
+            vy += sy0 * wz[c];
+            vz += s0 * dwz[c];
+        }
     }
     // the weights sum to 1: n = ref + sum (c - ref) w keeps the rounding on
     // the perturbation; the gradient weights sum to 0

@@ -967,8 +967,9 @@ struct simpler_scene {
     float4 *d_jtab = nullptr;  // Bessel J_m / J_m' table (see bessel_tab)
     int jtab_n = 0;
     float *d_spline = nullptr;  // coefficients minus spline_ref
+    float *d_spline_pad = nullptr;  // the same with the mirrored halo
     float spline_ref = 0.f;
-    int sN[3] = {0, 0, 0};
+    int sN[3] = {0, 0, 0}, sP[3] = {0, 0, 0};
     float sxmin[3] = {0, 0, 0}, sxmax[3] = {0, 0, 0};
     float spline_min = 0.f;
     int batches = 0;  // batches built on this scene
@@ -1097,9 +1098,11 @@ static DevScene make_dev_scene(const simpler_scene *sc, int64_t n_total, uint64_
     S.src_phase = sc->d_phase;
     for (int i = 0; i < 8; ++i) S.coeff_w[i] = sc->coeff_w[i];
     S.spline = sc->d_spline;
+    S.spline_pad = sc->d_spline_pad;
     S.spline_ref = sc->spline_ref;
     for (int i = 0; i < 3; ++i) {
         S.sN[i] = sc->sN[i];
+        S.sP[i] = sc->sP[i];
         S.sxmin[i] = sc->sxmin[i];
         S.sxres[i] = sc->sN[i] > 1 ? (float)(sc->sN[i] - 1) / (sc->sxmax[i] - sc->sxmin[i]) : 0.f;
     }
@@ -1533,6 +1536,7 @@ int simpler_scene_destroy(simpler_scene *sc) {
     (void)hipFree(sc->d_src);
     (void)hipFree(sc->d_phase);
     (void)hipFree(sc->d_spline);
+    (void)hipFree(sc->d_spline_pad);
     (void)hipFree(sc->d_jtab);
     delete sc;
     return SIMPLER_OK;
@@ -1597,6 +1601,43 @@ int simpler_scene_set_sources(simpler_scene *sc, int n_src, const float *pos, co
 
 static int check_grid(simpler_ctx *ctx, const int32_t dims[3], const float *xmin, const float *xmax);
 
+// The coefficient grid (N0 x N1 x N2, x fastest) copied into P = N + 2 h + 1
+// cells per axis, padded cell k holding the coefficient at the mirrored index
+// of k - h (h = kSplineHalo).
+__global__ void k_spline_pad(const float *src, int N0, int N1, int N2, float *dst, int P0, int P1, int P2) {
+    const long total = (long)P0 * P1 * P2;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const int kx = (int)(i % P0);
+        const long r = i / P0;
+        const int ky = (int)(r % P1), kz = (int)(r / P1);
+        dst[i] = src[((long)mirror_idx(kz - kSplineHalo, N2) * N1 + mirror_idx(ky - kSplineHalo, N1)) * N0 +
+                     mirror_idx(kx - kSplineHalo, N0)];
+    }
+}
+
+// Builds the padded copy of the scene's coefficients (sc->sN set, d_spline
+// filled); a scene without it evaluates through per-tap mirror wrapping.
+static int build_spline_pad(simpler_scene *sc) {
+    simpler_ctx *ctx = sc->ctx;
+    (void)hipFree(sc->d_spline_pad);
+    sc->d_spline_pad = nullptr;
+    int P[3];
+    for (int i = 0; i < 3; ++i) P[i] = sc->sN[i] + 2 * kSplineHalo + 1;
+    const size_t np = (size_t)P[0] * P[1] * P[2];
+    HIP_TRY(ctx, hipMalloc(&sc->d_spline_pad, np * sizeof(float)));
+    for (int i = 0; i < 3; ++i) sc->sP[i] = P[i];
+    const unsigned blocks = (unsigned)std::min<size_t>((np + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_spline_pad, dim3(blocks), dim3(256), 0, ctx->cur, sc->d_spline, sc->sN[0], sc->sN[1],
+                       sc->sN[2], sc->d_spline_pad, P[0], P[1], P[2]);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        (void)hipFree(sc->d_spline_pad);
+        sc->d_spline_pad = nullptr;
+        return fail(ctx, SIMPLER_ERR_HIP, std::string("spline pad: ") + hipGetErrorString(e));
+    }
+    return SIMPLER_OK;
+}
+
 int simpler_scene_set_spline(simpler_scene *sc, const float *coeffs, const int32_t dims[3], const float xmin[3],
                              const float xmax[3]) {
     if (!sc) return SIMPLER_ERR_INVALID;
@@ -1619,6 +1660,8 @@ int simpler_scene_set_spline(simpler_scene *sc, const float *coeffs, const int32
     HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
     (void)hipFree(sc->d_spline);
     sc->d_spline = nullptr;
+    (void)hipFree(sc->d_spline_pad);
+    sc->d_spline_pad = nullptr;
     HIP_TRY(ctx, hipMalloc(&sc->d_spline, n * sizeof(float)));
     HIP_TRY(ctx, hipMemcpy(sc->d_spline, rel.data(), n * sizeof(float), hipMemcpyHostToDevice));
     sc->spline_ref = ref;
@@ -1628,7 +1671,7 @@ int simpler_scene_set_spline(simpler_scene *sc, const float *coeffs, const int32
         sc->sxmin[i] = xmin[i];
         sc->sxmax[i] = xmax[i];
     }
-    return SIMPLER_OK;
+    return build_spline_pad(sc);
 }
 
 static int check_grid(simpler_ctx *ctx, const int32_t dims[3], const float *xmin, const float *xmax) {
@@ -1667,6 +1710,8 @@ int simpler_scene_set_spline_samples(simpler_scene *sc, const float *samples, in
     if (int rc = prefilter_buffer(ctx)) return rc;
     (void)hipFree(sc->d_spline);
     sc->d_spline = nullptr;
+    (void)hipFree(sc->d_spline_pad);
+    sc->d_spline_pad = nullptr;
     float *d = nullptr;
     HIP_TRY(ctx, hipMalloc(&d, n * sizeof(float)));
     // the shift (stored as spline_ref, added back by the field evaluation) is
@@ -1702,7 +1747,7 @@ int simpler_scene_set_spline_samples(simpler_scene *sc, const float *samples, in
         sc->sxmin[i] = xmin[i];
         sc->sxmax[i] = xmax[i];
     }
-    return SIMPLER_OK;
+    return build_spline_pad(sc);
 }
 
 __global__ void k_add_f32(const float *a, float *out, size_t n, float v) {
