@@ -520,30 +520,22 @@ __device__ __forceinline__ void field_eval_h<kSpline>(const DevScene &S, const F
     bspline_w2(gz - fz, wz, dwz, ddz);
     const int ix = (int)fx - 1, iy = (int)fy - 1, iz = (int)fz - 1;
     const int NX = S.sN[0], NY = S.sN[1], NZ = S.sN[2];
-    int xi[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) xi[q] = mirror_idx(ix + q, NX);
     // s_ab: derivative orders a in x, b in y after the x and y passes
     float v = 0, vx = 0, vy = 0, vz = 0, vxx = 0, vyy = 0, vzz = 0, vxy = 0, vxz = 0, vyz = 0;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int zi = mirror_idx(iz + c, NZ);
-        float s00 = 0, s10 = 0, s01 = 0, s20 = 0, s02 = 0, s11 = 0;
-#pragma unroll
-        for (int bq = 0; bq < 4; ++bq) {
-            const int yi = mirror_idx(iy + bq, NY);
-            const float *row = S.spline + ((long)zi * NY + yi) * NX;
-            const float c0 = row[xi[0]], c1 = row[xi[1]], c2 = row[xi[2]], c3 = row[xi[3]];
-            const float a0 = c0 * wx[0] + c1 * wx[1] + c2 * wx[2] + c3 * wx[3];
-            const float a1 = c0 * dwx[0] + c1 * dwx[1] + c2 * dwx[2] + c3 * dwx[3];
-            const float a2 = c0 * ddx[0] + c1 * ddx[1] + c2 * ddx[2] + c3 * ddx[3];
-            s00 += a0 * wy[bq];
-            s10 += a1 * wy[bq];
-            s01 += a0 * dwy[bq];
-            s20 += a2 * wy[bq];
-            s02 += a0 * ddy[bq];
-            s11 += a1 * dwy[bq];
-        }
+    // one row of 4 x-taps into the six (x, y) derivative sums of its slab
+    auto row_acc = [&](float c0, float c1, float c2, float c3, int bq, float &s00, float &s10, float &s01,
+                       float &s20, float &s02, float &s11) {
+        const float a0 = c0 * wx[0] + c1 * wx[1] + c2 * wx[2] + c3 * wx[3];
+        const float a1 = c0 * dwx[0] + c1 * dwx[1] + c2 * dwx[2] + c3 * dwx[3];
+        const float a2 = c0 * ddx[0] + c1 * ddx[1] + c2 * ddx[2] + c3 * ddx[3];
+        s00 += a0 * wy[bq];
+        s10 += a1 * wy[bq];
+        s01 += a0 * dwy[bq];
+        s20 += a2 * wy[bq];
+        s02 += a0 * ddy[bq];
+        s11 += a1 * dwy[bq];
+    };
+    auto slab_acc = [&](int c, float s00, float s10, float s01, float s20, float s02, float s11) {
         v += s00 * wz[c];
         vx += s10 * wz[c];
         vy += s01 * wz[c];
@@ -554,6 +546,40 @@ __device__ __forceinline__ void field_eval_h<kSpline>(const DevScene &S, const F
         vxy += s11 * wz[c];
         vxz += s10 * dwz[c];
         vyz += s01 * dwz[c];
+    };
+    // taps inside the padded grid: fixed strides from one corner (see
+    // field_eval<kSpline>)
+    const int jx = ix + kSplineHalo, jy = iy + kSplineHalo, jz = iz + kSplineHalo;
+    if (S.spline_pad && (unsigned)jx <= (unsigned)(S.sP[0] - 4) && (unsigned)jy <= (unsigned)(S.sP[1] - 4) &&
+        (unsigned)jz <= (unsigned)(S.sP[2] - 4)) {
+        const long row = S.sP[0], slab = (long)S.sP[0] * S.sP[1];
+        const float *base = S.spline_pad + (long)jz * slab + (long)jy * row + jx;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float s00 = 0, s10 = 0, s01 = 0, s20 = 0, s02 = 0, s11 = 0;
+#pragma unroll
+            for (int bq = 0; bq < 4; ++bq) {
+                const float *r = base + c * slab + bq * row;
+                row_acc(r[0], r[1], r[2], r[3], bq, s00, s10, s01, s20, s02, s11);
+            }
+            slab_acc(c, s00, s10, s01, s20, s02, s11);
+        }
+    } else {
+        int xi[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) xi[q] = mirror_idx(ix + q, NX);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int zi = mirror_idx(iz + c, NZ);
+            float s00 = 0, s10 = 0, s01 = 0, s20 = 0, s02 = 0, s11 = 0;
+#pragma unroll
+            for (int bq = 0; bq < 4; ++bq) {
+                const int yi = mirror_idx(iy + bq, NY);
+                const float *r = S.spline + ((long)zi * NY + yi) * NX;
+                row_acc(r[xi[0]], r[xi[1]], r[xi[2]], r[xi[3]], bq, s00, s10, s01, s20, s02, s11);
+            }
+            slab_acc(c, s00, s10, s01, s20, s02, s11);
+        }
     }
     const float rx = S.sxres[0], ry = S.sxres[1], rz = S.sxres[2];
     n = S.spline_ref + v;
