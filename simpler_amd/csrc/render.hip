@@ -1629,7 +1629,10 @@ static int build_spline_pad(simpler_scene *sc) {
     const unsigned blocks = (unsigned)std::min<size_t>((np + 255) / 256, 65536);
     hipLaunchKernelGGL(k_spline_pad, dim3(blocks), dim3(256), 0, ctx->cur, sc->d_spline, sc->sN[0], sc->sN[1],
                        sc->sN[2], sc->d_spline_pad, P[0], P[1], P[2]);
-    const hipError_t e = hipGetLastError();
+    // complete before returning, like the rest of the upload: a render that
+    // follows on another stream must not overtake it
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->cur);
     if (e != hipSuccess) {
         (void)hipFree(sc->d_spline_pad);
         sc->d_spline_pad = nullptr;
