@@ -363,31 +363,74 @@ def prefilter_bench(args):
                                   sample=f"oracle build3d (FP64) of a {m}^3 grid, {cpu_s:.2f} s"))
 
 
+def host_info():
+    """The host the CPU baseline ran on: logical CPUs, the CPUs this process
+    may use, and the CPU model."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(nproc=os.cpu_count(), affinity=len(os.sched_getaffinity(0)), model=model)
+
+
 def cpu_baseline(args):
+    """The CPU restatement of the reference path (4 field evaluations per ER
+    step, glibc jn, FP32, the reference's -O3 -ffast-math -fopenmp flags) on
+    this host, one photon per OpenMP work item.
+
+    The per-photon cost of the reference algorithm is heavy-tailed (a
+    next-event ray trapped by total internal reflection can march up to the
+    10 km LargeDist), so the wall time of a bounded sample is set by its
+    longest photon while the other threads idle.  The rate reported is the
+    sample's photons over its CPU-seconds per thread (photons x threads /
+    summed per-thread tracing time), the rate a long render keeps while every
+    thread is busy; the sample's own wall-clock rate is reported beside it."""
     from oracle import oracle as orc
     from simpler_amd import scenes
 
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
+    host = host_info()
     d, tex, res = scenes.bessel_cube_full(0)
     sc = orc.OracleScene(d, tex, fast=True)
-    n = args.cpu_photons
+    n = args.cpu_photons or 128 * threads
     t = time.perf_counter()
     _, st = sc.render(n, res, seed=1, offset=0, n_total=N_TOTAL, precision=32, threads=threads, cutoff=False)
     dt = time.perf_counter() - t
-    eprint(f"cpu baseline: {n} photons in {dt:.1f}s on {threads} threads")
-    nc = args.cpu_photons_cutoff
-    t = time.perf_counter()
-    _, stc = sc.render(nc, res, seed=1, offset=0, n_total=N_TOTAL, precision=32, threads=threads, cutoff=True)
-    dtc = time.perf_counter() - t
-    eprint(f"cpu with dead-path cutoff: {nc} photons in {dtc:.1f}s")
-    return dict(value=n / dt, unit="photon paths/s", cores=threads, kind="port",
-                sample=(f"photons 0-{n - 1} (seed 1) of the mode-0 1e9-photon render, reference algorithm in "
-                        f"full (4 field evaluations per ER step, glibc jn, no dead-path cutoff), FP32, "
-                        f"g++ -O3 -ffast-math -fopenmp, {threads} threads; {st['er_steps']} ER steps in {dt:.1f} s"),
-                er_steps_per_s=st["er_steps"] / dt,
-                same_algorithm_as_gpu=dict(value=nc / dtc, photons=nc, seconds=dtc,
-                                           er_steps_per_s=stc["er_steps"] / dtc,
-                                           note="same CPU code with the dead-path cutoff the GPU uses"))
+    busy = st["busy_seconds"]
+    eprint(f"cpu baseline: {n} photons in {dt:.1f}s wall, {busy:.1f} CPU-s on {threads} threads")
+    # the same CPU code with the dead-path cutoff the GPU uses: three
+    # disjoint photon blocks, median rate
+    nc = args.cpu_photons_cutoff or 128 * threads
+    cut = []
+    for k in range(3):
+        t = time.perf_counter()
+        _, stc = sc.render(nc, res, seed=1, offset=n + k * nc, n_total=N_TOTAL, precision=32, threads=threads,
+                           cutoff=True)
+        w = time.perf_counter() - t
+        cut.append(dict(rate=nc * threads / stc["busy_seconds"], wall_rate=nc / w, seconds=w,
+                        er_steps=stc["er_steps"], busy_seconds=stc["busy_seconds"]))
+    cut.sort(key=lambda c: c["rate"])
+    med = cut[1]
+    eprint(f"cpu with dead-path cutoff: median {med['rate']:.4g} paths/s over 3 blocks of {nc}")
+    return dict(value=n * threads / busy, unit="photon paths/s", cores=threads, kind="port",
+                sample=(f"photons 0-{n - 1} (seed 1) of the mode-0 1e9-photon render of the 5 um / 256x256x128 "
+                        f"scene, reference algorithm in full (4 field evaluations per ER step, glibc jn, no "
+                        f"dead-path cutoff), FP32, g++ -O3 -ffast-math -fopenmp, {threads} threads, one photon per "
+                        f"work item; {st['er_steps']} ER steps, {busy:.1f} CPU-s, {dt:.1f} s wall. value = photons x "
+                        f"threads / CPU-s (all threads busy); wall_rate = photons / wall of this sample"),
+                wall_rate=n / dt, single_thread_rate=n / busy, er_steps_per_s_per_thread=st["er_steps"] / busy,
+                er_steps_per_photon=st["er_steps"] / n, host=host,
+                same_algorithm_as_gpu=dict(value=med["rate"], photons_per_block=nc, blocks=3,
+                                           wall_rate=med["wall_rate"], seconds=med["seconds"],
+                                           er_steps_per_s_per_thread=med["er_steps"] / med["busy_seconds"],
+                                           er_steps_per_photon=med["er_steps"] / nc,
+                                           note="same CPU code with the dead-path cutoff the GPU uses; median of 3 "
+                                                "disjoint photon blocks"))
 
 
 def profile_prefilter(args):
@@ -449,8 +492,8 @@ def main():
     ap.add_argument("--tune", default="", help="block_size,steps_per_chunk,blocks_per_cu (skips the autotuner)")
     ap.add_argument("--autotune-photons", type=int, default=-1,
                     help="photons per autotuner trial (default: the step's chunk size, at most 2^24; 0: no autotuning)")
-    ap.add_argument("--cpu-photons", type=int, default=96)
-    ap.add_argument("--cpu-photons-cutoff", type=int, default=768)
+    ap.add_argument("--cpu-photons", type=int, default=0, help="reference-algorithm sample (default 128 x threads)")
+    ap.add_argument("--cpu-photons-cutoff", type=int, default=0, help="cutoff blocks (default 128 x threads)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-prefilter", action="store_true")

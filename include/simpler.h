@@ -162,7 +162,11 @@ int simpler_ctx_destroy(simpler_ctx *ctx);
 /* Run the context's work on a caller's HIP stream (hipStream_t passed as
  * void*).  NULL is the HIP null stream, which orders with every blocking
  * stream (e.g. PyTorch's default stream); a new context uses its own
- * non-blocking stream, whose handle simpler_ctx_own_stream returns. */
+ * non-blocking stream, whose handle simpler_ctx_own_stream returns.  The
+ * switch is ordered: work issued afterwards on the new stream starts after
+ * the work already queued on the previous stream (an event wait, no host
+ * synchronisation), because all launches of a context share its work queue,
+ * counters and adjoint buffers. */
 int simpler_ctx_set_stream(simpler_ctx *ctx, void *hip_stream);
 void *simpler_ctx_get_stream(simpler_ctx *ctx);
 void *simpler_ctx_own_stream(simpler_ctx *ctx);
@@ -378,8 +382,10 @@ int simpler_adjoint_rays(simpler_scene *scene, int64_t n_rays, uint64_t seed,
                          simpler_adjoint_stats *stats_host);
 
 /* Tunables of the persistent march kernel (picked by the autotuner):
- * block_size in {64,128,256,512}, steps_per_chunk >= 1 (march steps between
- * event phases), blocks_per_cu >= 1.  0 keeps the current value. */
+ * block_size in {64,128,256} (the kernels are compiled for at most 256
+ * lanes per workgroup), steps_per_chunk in [1,4096] (march steps between
+ * event phases), blocks_per_cu in [1,64].  0 keeps the current value; other
+ * values fail with SIMPLER_ERR_INVALID. */
 int simpler_ctx_set_tuning(simpler_ctx *ctx, int block_size,
                            int steps_per_chunk, int blocks_per_cu);
 int simpler_ctx_get_tuning(simpler_ctx *ctx, int *block_size,

@@ -1011,9 +1011,11 @@ static R dead_path_cut(const Scene<R> &S) {
     return (R)((double)d.pathlength_range[1] + margin);
 }
 
+/* busy (optional): per-thread seconds spent tracing photons, summed over
+ * the threads (CPU-seconds of the render, independent of load balance). */
 template <class R>
 static int render(const Scene<R> &S, int64_t n, uint64_t seed, uint64_t offset, int cutoff,
-                  double *img, int X, int Y, int Z, simpler_stats *st, int threads) {
+                  double *img, int X, int Y, int Z, simpler_stats *st, int threads, double *busy) {
     const R Lcut = cutoff ? dead_path_cut(S) : (R)INFINITY;
     const long npix = (long)X * Y * Z;
     int nt = 1;
@@ -1022,11 +1024,13 @@ static int render(const Scene<R> &S, int64_t n, uint64_t seed, uint64_t offset, 
 #endif
     std::vector<std::vector<double>> imgs(nt, std::vector<double>(npix, 0.0));
     std::vector<Counters> cs(nt);
-#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+    std::vector<double> tbusy(nt, 0.0);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
     for (int64_t i = 0; i < n; ++i) {
         int tid = 0;
 #ifdef _OPENMP
         tid = omp_get_thread_num();
+        const double t0 = omp_get_wtime();
 #endif
         Sampler smp;
         smp.key[0] = (uint32_t)seed;
@@ -1036,6 +1040,13 @@ static int render(const Scene<R> &S, int64_t n, uint64_t seed, uint64_t offset, 
         smp.draw = 0;
         Tracer<R> t{S, cs[tid], &smp, {0, 0}, Lcut, false};
         t.photon(imgs[tid].data(), X, Y, Z);
+#ifdef _OPENMP
+        tbusy[tid] += omp_get_wtime() - t0;
+#endif
+    }
+    if (busy) {
+        *busy = 0.0;
+        for (double b : tbusy) *busy += b;
     }
     for (int t = 0; t < nt; ++t)
         for (long k = 0; k < npix; ++k) img[k] += imgs[t][k];
@@ -1062,7 +1073,7 @@ static int trace_rays(const Scene<R> &S, int64_t n, uint64_t seed, uint64_t offs
 #ifdef _OPENMP
     nt = threads > 0 ? threads : omp_get_num_procs();
 #endif
-#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
     for (int64_t i = 0; i < n; ++i) {
         Counters c;
         Sampler smp;
@@ -1219,16 +1230,16 @@ void oracle_scene_set_spline(void *hv, const float *coeffs, const int32_t dims[3
  * 0 runs the reference algorithm in full. */
 int oracle_render(void *hv, int precision, int64_t n_photons, int64_t n_total, uint64_t seed,
                   uint64_t offset, int cutoff, double *image, int X, int Y, int Z, simpler_stats *st,
-                  int threads) {
+                  int threads, double *busy_seconds) {
     const OracleSceneHandle &h = *(OracleSceneHandle *)hv;
     if (precision == 64) {
         Scene<double> S;
         fill_scene(S, h, n_total);
-        return render(S, n_photons, seed, offset, cutoff, image, X, Y, Z, st, threads);
+        return render(S, n_photons, seed, offset, cutoff, image, X, Y, Z, st, threads, busy_seconds);
     }
     Scene<float> S;
     fill_scene(S, h, n_total);
-    return render(S, n_photons, seed, offset, cutoff, image, X, Y, Z, st, threads);
+    return render(S, n_photons, seed, offset, cutoff, image, X, Y, Z, st, threads, busy_seconds);
 }
 
 /* Single path functions in FP64, for the distribution / closed-form pins. */

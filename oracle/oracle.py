@@ -47,7 +47,7 @@ def lib(fast=False):
     L.oracle_scene_set_sources.argtypes = [P, C.c_int, fp, fp, fp, C.c_int, fp]
     L.oracle_scene_set_spline.argtypes = [P, fp, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.oracle_render.argtypes = [P, C.c_int, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, C.c_int,
-                                dp, C.c_int, C.c_int, C.c_int, C.POINTER(Stats), C.c_int]
+                                dp, C.c_int, C.c_int, C.c_int, C.POINTER(Stats), C.c_int, C.POINTER(C.c_double)]
     L.oracle_dead_path_cut.argtypes = [P]
     L.oracle_dead_path_cut.restype = C.c_double
     L.oracle_trace_rays.argtypes = [P, C.c_int, C.c_int64, C.c_uint64, C.c_uint64, dp, dp, dp, ip, C.c_int]
@@ -127,13 +127,17 @@ class OracleScene:
 
     def render(self, n, res, seed=1, offset=0, n_total=None, precision=32, threads=0, cutoff=False):
         """Render photons [offset, offset+n) of an n_total-photon render;
-        returns the image [z][y][x] (float64) and the event counters."""
+        returns the image [z][y][x] (float64) and the event counters (plus
+        'busy_seconds': the CPU-seconds the threads spent tracing)."""
         X, Y, Z = res
         img = np.zeros(X * Y * Z, np.float64)
         st = Stats()
+        busy = C.c_double()
         self.L.oracle_render(self.h, precision, n, n_total or n, seed, offset, int(cutoff), img, X, Y, Z,
-                             C.byref(st), threads)
-        return img.reshape(Z, Y, X), st.as_dict()
+                             C.byref(st), threads, C.byref(busy))
+        out = st.as_dict()
+        out["busy_seconds"] = busy.value
+        return img.reshape(Z, Y, X), out
 
     def cdf_sample(self, u):
         """Texel indices that the source's texture CDF maps the uniforms u to."""

@@ -9,6 +9,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <atomic>
+#include <memory>
 #include <mutex>
 #include <sstream>
 
@@ -215,6 +217,16 @@ std::vector<float> prefilter3d(const std::vector<float> &data, const int N[3]) {
 
 namespace scn {
 
+// Content revisions are drawn from one process-wide counter, so a revision
+// names the texture / source lattice / grid content of a scene uniquely (a
+// copy of a scene shares its content and its revision); descriptor-only
+// changes (set_f_u, set_n_max, mutable_desc) keep the revision and are found
+// by comparing descriptors.
+static uint64_t next_revision() {
+    static std::atomic<uint64_t> counter{0};
+    return ++counter;
+}
+
 template <template <typename> class V>
 void Scene<V>::common(Float ior, const V<Float> &blockL, const V<Float> &blockR, const V<Float> &lightOrigin,
                       const V<Float> &lightDir, const Float &halfThetaLimit, const std::string &lightTextureFile,
@@ -267,7 +279,7 @@ void Scene<V>::common(Float ior, const V<Float> &blockL, const V<Float> &blockR,
     d.sgap_begin_loc_x = SgapBeginLocX;
     d.flags = SIMPLER_FLAGS_DEFAULT;
     m_texture = image::readPFM(lightTextureFile, m_tex_x, m_tex_y);
-    ++m_revision;
+    m_revision = next_revision();
 }
 
 template <template <typename> class V>
@@ -324,6 +336,7 @@ Scene<V>::Scene(Float ior, const V<Float> &blockL, const V<Float> &blockR, const
     // is uploaded (simpler_scene_set_spline_samples)
     m_spline_samples = spline::readVol(rifgridFile);
     m_desc.field_kind = SIMPLER_FIELD_SPLINE;
+    m_revision = next_revision();
 }
 
 template <template <typename> class V>
@@ -412,18 +425,16 @@ void Scene<V>::rebuild_sources() {
     for (int c = 1; c <= m_n_coeff; ++c)
         m_src_coeff.push_back((float)(m_n_scaling * 4.0 * ((c + 1) % 2 == 0 ? 1.0 : -1.0) /
                                       (std::pow(c * M_PI, 2) * nsrc)));
-    ++m_revision;
+    m_revision = next_revision();
 }
 
 template <template <typename> class V>
 void Scene<V>::set_f_u(Float value) {
     m_desc.f_u = value;
-    ++m_revision;
 }
 template <template <typename> class V>
 void Scene<V>::set_n_max(Float value) {
     m_desc.n_max = value;
-    ++m_revision;
 }
 template <template <typename> class V>
 void Scene<V>::set_n_scaling(Float value) {
@@ -440,9 +451,24 @@ void Scene<V>::set_phase2(Float value) {
     m_phase2 = value;
     if (m_desc.field_kind == SIMPLER_FIELD_SOURCES) rebuild_sources();
 }
+// scene.h:141-164: transmission into the medium of the source direction,
+// ior^2 (1 - F), when Fresnel weighting is on; 1 otherwise (the reference's
+// CMake build compiles Fresnel out, which SIMPLER_FLAG_USE_FRESNEL undoes).
 template <template <typename> class V>
 Float Scene<V>::getFresnelTrans() const {
-    return 1.0f;  // Fresnel compiled out in the reference build (scene.h:157-159)
+    const simpler_scene_desc &d = m_desc;
+    if (!(d.flags & SIMPLER_FLAG_USE_FRESNEL) || !(d.ior > 1.0f)) return 1.0f;
+    const float ry = d.light_dir[1] / d.ior, rz = d.light_dir[2] / d.ior;
+    float rx = std::sqrt(1.0f - (ry * ry + rz * rz));
+    if (d.light_dir[0] < 0.0f) rx = -rx;
+    const float eta = d.ior, cI = d.light_dir[0];
+    float F = 0.0f;
+    if (!(std::fabs(eta - 1.0f) < 1.19209290e-7f * std::max(1.0f, std::fabs(eta)))) {  // util.h:42-56
+        const float Rs = (cI - eta * rx) / (cI + eta * rx);
+        const float Rp = (rx - eta * cI) / (rx + eta * cI);
+        F = 0.5f * (Rs * Rs + Rp * Rp);
+    }
+    return eta * eta * (1.0f - F);
 }
 
 template <template <typename> class V>
@@ -472,11 +498,27 @@ simpler_ctx *engine_context() {
     return ctx;
 }
 
+// The device copy of the last scene a renderer rendered: a render of the
+// same content (same revision) reuses it, with at most a descriptor update
+// (a cheap call unless the Bessel table must be rebuilt), so a parameter
+// sweep through the mutators does not re-upload the texture CDF, the source
+// lattice or the grid on every renderImage.
+struct DeviceSceneCache {
+    std::mutex mu;
+    simpler_scene *sc = nullptr;
+    uint64_t revision = 0;
+    simpler_scene_desc desc{};
+    ~DeviceSceneCache() {
+        if (sc) simpler_scene_destroy(sc);
+    }
+};
+
 template <template <typename> class V>
 Renderer<V>::Renderer(const int maxDepth, const Float maxPathlength, const bool useDirect,
                       const bool useAngularSampling, const int64_t threads)
     : m_maxDepth(maxDepth), m_maxPathlength(maxPathlength), m_useDirect(useDirect),
-      m_useAngularSampling(useAngularSampling), m_threads(threads) {}
+      m_useAngularSampling(useAngularSampling), m_threads(threads),
+      m_cache(std::make_shared<DeviceSceneCache>()) {}
 
 template <template <typename> class V>
 simpler_scene_desc Renderer<V>::full_desc(const med::Medium &medium, const scn::Scene<V> &scene) const {
@@ -497,23 +539,42 @@ void Renderer<V>::renderChunk(image::SmallImage &img0, const med::Medium &medium
                               simpler_stats *stats) const {
     simpler_ctx *ctx = engine_context();
     const simpler_scene_desc d = full_desc(medium, scene);
-    simpler_scene *sc = nullptr;
-    check(ctx, simpler_scene_create(ctx, &d, scene.texture().data(), scene.tex_x(), scene.tex_y(), &sc),
-          "scene upload");
-    struct Guard {
-        simpler_scene *s;
-        ~Guard() { simpler_scene_destroy(s); }
-    } guard{sc};
-    if (d.field_kind == SIMPLER_FIELD_SOURCES)
-        check(ctx,
-              simpler_scene_set_sources(sc, (int)scene.source_amp().size(), scene.source_pos().data(),
-                                        scene.source_amp().data(), scene.source_phase().data(),
-                                        (int)scene.source_coeff().size(), scene.source_coeff().data()),
-              "sources upload");
-    if (d.field_kind == SIMPLER_FIELD_SPLINE) {
-        const spline::VolGrid &g = scene.spline_samples();
-        check(ctx, simpler_scene_set_spline_samples(sc, g.data.data(), 0, g.N, g.xmin, g.xmax), "spline upload");
+    DeviceSceneCache &cache = *m_cache;
+    std::lock_guard<std::mutex> lock(cache.mu);
+    if (cache.sc && cache.revision == scene.revision() && cache.desc.field_kind == d.field_kind) {
+        if (std::memcmp(&cache.desc, &d, sizeof(d)) != 0) {
+            check(ctx, simpler_scene_update(cache.sc, &d), "scene update");
+            cache.desc = d;
+        }
+    } else {
+        if (cache.sc) simpler_scene_destroy(cache.sc);
+        cache.sc = nullptr;
+        simpler_scene *fresh = nullptr;
+        check(ctx, simpler_scene_create(ctx, &d, scene.texture().data(), scene.tex_x(), scene.tex_y(), &fresh),
+              "scene upload");
+        struct Guard {
+            simpler_scene *s;
+            ~Guard() {
+                if (s) simpler_scene_destroy(s);
+            }
+        } guard{fresh};
+        if (d.field_kind == SIMPLER_FIELD_SOURCES)
+            check(ctx,
+                  simpler_scene_set_sources(fresh, (int)scene.source_amp().size(), scene.source_pos().data(),
+                                            scene.source_amp().data(), scene.source_phase().data(),
+                                            (int)scene.source_coeff().size(), scene.source_coeff().data()),
+                  "sources upload");
+        if (d.field_kind == SIMPLER_FIELD_SPLINE) {
+            const spline::VolGrid &g = scene.spline_samples();
+            check(ctx, simpler_scene_set_spline_samples(fresh, g.data.data(), 0, g.N, g.xmin, g.xmax),
+                  "spline upload");
+        }
+        guard.s = nullptr;
+        cache.sc = fresh;
+        cache.revision = scene.revision();
+        cache.desc = d;
     }
+    simpler_scene *sc = cache.sc;
     check(ctx,
           simpler_render_host(sc, numPhotons, nTotal, m_seed, (uint64_t)offset, img0.data(), img0.get_x_res(),
                               img0.get_y_res(), img0.getZRes(), accumulate ? 1 : 0, stats),

@@ -266,6 +266,8 @@ private:
 
 namespace photon {
 
+struct DeviceSceneCache;
+
 // reference photon.h:24-116 (forward path only)
 template <template <typename> class vector_type>
 class Renderer {
@@ -302,6 +304,7 @@ private:
     bool m_useDirect, m_useAngularSampling;
     int64_t m_threads;
     uint64_t m_seed = 0;
+    std::shared_ptr<DeviceSceneCache> m_cache;  // device copy of the last scene rendered
 };
 
 // the process-wide engine context (device SIMPLER_DEVICE, default 0)

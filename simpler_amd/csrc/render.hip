@@ -951,6 +951,7 @@ struct simpler_ctx {
     size_t log_bytes = 0;
     unsigned long long *d_adj = nullptr;  // adjoint objective (double) + counters
     unsigned *d_pref = nullptr;           // prefilter: non-finite flag, minimum key
+    hipEvent_t handoff = nullptr;         // orders a stream switch after the old stream's work
     std::string err;
 };
 
@@ -1344,7 +1345,9 @@ int simpler_ctx_create(int device, simpler_ctx **out) {
     }
     (void)hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, device);
     if (hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->handoff, hipEventDisableTiming) != hipSuccess ||
         hipMalloc(&ctx->d_scratch, 16 * sizeof(unsigned long long)) != hipSuccess) {
+        if (ctx->handoff) (void)hipEventDestroy(ctx->handoff);
         if (ctx->own) (void)hipStreamDestroy(ctx->own);
         delete ctx;
         return SIMPLER_ERR_HIP;
@@ -1363,14 +1366,24 @@ int simpler_ctx_destroy(simpler_ctx *ctx) {
     (void)hipFree(ctx->d_log);
     (void)hipFree(ctx->d_adj);
     (void)hipFree(ctx->d_pref);
+    (void)hipEventDestroy(ctx->handoff);
     (void)hipStreamDestroy(ctx->own);
     delete ctx;
     return SIMPLER_OK;
 }
 
+// Every launch of a context shares its work queue, counters and adjoint
+// buffers, so work on the new stream must not start before the work already
+// queued on the old one has finished: the new stream waits on an event
+// recorded on the old stream (no host synchronisation).
 int simpler_ctx_set_stream(simpler_ctx *ctx, void *stream) {
     if (!ctx) return SIMPLER_ERR_INVALID;
-    ctx->cur = (hipStream_t)stream;
+    const hipStream_t s = (hipStream_t)stream;
+    if (s == ctx->cur) return SIMPLER_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipEventRecord(ctx->handoff, ctx->cur));
+    HIP_TRY(ctx, hipStreamWaitEvent(s, ctx->handoff, 0));
+    ctx->cur = s;
     return SIMPLER_OK;
 }
 
@@ -1616,7 +1629,9 @@ __global__ void k_spline_pad(const float *src, int N0, int N1, int N2, float *ds
 }
 
 // Builds the padded copy of the scene's coefficients (sc->sN set, d_spline
-// filled); a scene without it evaluates through per-tap mirror wrapping.
+// filled); a scene without it evaluates through per-tap mirror wrapping,
+// which gives the same values, so a copy that cannot be allocated is left
+// out (and the allocation error cleared) instead of failing the upload.
 static int build_spline_pad(simpler_scene *sc) {
     simpler_ctx *ctx = sc->ctx;
     (void)hipFree(sc->d_spline_pad);
@@ -1624,7 +1639,11 @@ static int build_spline_pad(simpler_scene *sc) {
     int P[3];
     for (int i = 0; i < 3; ++i) P[i] = sc->sN[i] + 2 * kSplineHalo + 1;
     const size_t np = (size_t)P[0] * P[1] * P[2];
-    HIP_TRY(ctx, hipMalloc(&sc->d_spline_pad, np * sizeof(float)));
+    if (hipMalloc(&sc->d_spline_pad, np * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        sc->d_spline_pad = nullptr;
+        return SIMPLER_OK;
+    }
     for (int i = 0; i < 3; ++i) sc->sP[i] = P[i];
     const unsigned blocks = (unsigned)std::min<size_t>((np + 255) / 256, 65536);
     hipLaunchKernelGGL(k_spline_pad, dim3(blocks), dim3(256), 0, ctx->cur, sc->d_spline, sc->sN[0], sc->sN[1],

@@ -20,6 +20,22 @@ class SimplerError(RuntimeError):
     pass
 
 
+class SimplerWarning(RuntimeWarning):
+    pass
+
+
+def _warn_dropped(st):
+    """The adjoint leaves out the contributions of photons whose segment log
+    overflowed (more than 512 main-path segments, or more than 64 face
+    reflections in one next-event / direct sub-path); their objective still
+    counts, so the gradient is biased by their share."""
+    if st.get("dropped", 0):
+        import warnings
+        warnings.warn(SimplerWarning(
+            f"adjoint: {st['dropped']} splat contribution(s) left out of the gradient (segment log full: "
+            f"more than 512 main-path segments or 64 sub-path reflections in one photon)"), stacklevel=3)
+
+
 def load_library():
     """Load libsimpler_hip.so (built in-tree by simpler_amd.build)."""
     global _LIB
@@ -98,8 +114,12 @@ def _fp(a):
 
 
 class Context:
-    """One GPU + one HIP stream (by default torch's current stream, so torch
-    events and kernels order with the renders)."""
+    """One GPU + one HIP stream.  By default the context follows torch's
+    current stream: every call first moves the context to the stream that is
+    current at that moment (an ordered switch, see simpler_ctx_set_stream),
+    so the tensors it returns are ready in the caller's stream order, also
+    inside `with torch.cuda.stream(s)`.  use_torch_stream=False keeps the
+    context's own stream; an explicit set_stream pins a stream."""
 
     def __init__(self, device=0, use_torch_stream=True):
         import torch
@@ -115,8 +135,19 @@ class Context:
         if rc != 0:
             raise SimplerError(f"simpler_ctx_create failed ({rc})")
         self.h = h
-        if use_torch_stream:
-            self.set_stream(torch.cuda.current_stream(device).cuda_stream)
+        self.follow_torch = bool(use_torch_stream)
+        self._bound = None
+        self._bind()
+
+    def _bind(self):
+        """Move the context to torch's current stream (when following it)."""
+        if not self.follow_torch:
+            return
+        import torch
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        if s != self._bound:
+            self.check(self.L.simpler_ctx_set_stream(self.h, C.c_void_p(s)))
+            self._bound = s
 
     def check(self, rc):
         if rc != 0:
@@ -129,10 +160,14 @@ class Context:
         if grid.dtype.itemsize != 4 or not grid.is_contiguous() or not grid.is_cuda or grid.numel() != int(
                 np.prod(dims)):
             raise SimplerError("grid must be a contiguous float32 CUDA tensor matching dims")
+        self._bind()
         self.check(self.L.simpler_spline_prefilter(self.h, C.c_void_p(grid.data_ptr()), (C.c_int32 * 3)(*dims)))
 
     def set_stream(self, stream_ptr):
+        """Pin the context to a HIP stream (it stops following torch's)."""
+        self.follow_torch = False
         self.check(self.L.simpler_ctx_set_stream(self.h, C.c_void_p(stream_ptr)))
+        self._bound = stream_ptr
 
     def synchronize(self):
         self.check(self.L.simpler_ctx_synchronize(self.h))
@@ -155,6 +190,7 @@ class Context:
         return dict(block_size=b.value, steps_per_chunk=s.value, blocks_per_cu=c.value)
 
     def read_stats(self):
+        self._bind()
         st = Stats()
         self.check(self.L.simpler_read_stats(self.h, C.byref(st)))
         return st.as_dict()
@@ -205,12 +241,14 @@ class Scene:
         amp = np.ascontiguousarray(amp, np.float32)
         phase = np.ascontiguousarray(phase, np.float32)
         coeff = np.ascontiguousarray(coeff, np.float32)
+        self.ctx._bind()
         self.ctx.check(self.L.simpler_scene_set_sources(self.h, len(amp), _fp(pos), _fp(amp), _fp(phase), len(coeff),
                                                         _fp(coeff)))
         self._sources = dict(pos=pos, amp=amp, phase=phase, coeff=coeff)
 
     def set_spline(self, coeffs, dims, xmin, xmax):
         c = np.ascontiguousarray(coeffs, np.float32).reshape(-1)
+        self.ctx._bind()
         self.ctx.check(self.L.simpler_scene_set_spline(self.h, _fp(c), (C.c_int32 * 3)(*dims),
                                                        (C.c_float * 3)(*xmin), (C.c_float * 3)(*xmax)))
         self._spline = (tuple(dims), tuple(xmin), tuple(xmax), c)
@@ -229,6 +267,7 @@ class Scene:
         count = keep.numel() if on_dev else keep.size
         if count != int(np.prod(dims)):
             raise SimplerError("samples do not match dims")
+        self.ctx._bind()
         self.ctx.check(self.L.simpler_scene_set_spline_samples(self.h, C.c_void_p(ptr), on_dev,
                                                                (C.c_int32 * 3)(*dims), (C.c_float * 3)(*xmin),
                                                                (C.c_float * 3)(*xmax)))
@@ -241,6 +280,7 @@ class Scene:
         if dims is None:
             raise SimplerError("the scene has no spline field")
         out = np.zeros(int(np.prod(dims)), np.float32)
+        self.ctx._bind()
         self.ctx.check(self.L.simpler_scene_get_spline(self.h, _fp(out)))
         return out
 
@@ -288,6 +328,7 @@ class Scene:
         if image.dtype != torch.float32 or not image.is_contiguous() or image.numel() != X * Y * Z:
             raise SimplerError("image must be a contiguous float32 tensor of X*Y*Z elements")
         st = Stats()
+        self.ctx._bind()
         self.ctx.check(self.L.simpler_render(self.h, n, n_total or n, seed, offset, C.c_void_p(image.data_ptr()), X,
                                              Y, Z, int(accumulate), C.byref(st) if stats else None))
         return image, (st.as_dict() if stats else None)
@@ -296,6 +337,7 @@ class Scene:
         X, Y, Z = res
         img = np.zeros(X * Y * Z, np.float32)
         st = Stats()
+        self.ctx._bind()
         self.ctx.check(self.L.simpler_render_host(self.h, n, n_total or n, seed, offset, _fp(img), X, Y, Z, 0,
                                                   C.byref(st)))
         return img.reshape(Z, Y, X), st.as_dict()
@@ -309,6 +351,7 @@ class Scene:
         d = torch.empty((n, 3), dtype=torch.float32, device=dev)
         L = torch.empty((n,), dtype=torch.float32, device=dev)
         s = torch.empty((n,), dtype=torch.int32, device=dev)
+        self.ctx._bind()
         self.ctx.check(self.L.simpler_trace_rays(self.h, n, seed, offset, C.c_void_p(p.data_ptr()),
                                                  C.c_void_p(d.data_ptr()), C.c_void_p(L.data_ptr()),
                                                  C.c_void_p(s.data_ptr())))
@@ -320,6 +363,7 @@ class Scene:
         the context.  Returns it with the best time (ms)."""
         b, s, c, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
         X, Y, Z = res
+        self.ctx._bind()
         self.ctx.check(self.L.simpler_autotune(self.h, n, X, Y, Z, C.byref(b), C.byref(s), C.byref(c), C.byref(ms)))
         return dict(block_size=b.value, steps_per_chunk=s.value, blocks_per_cu=c.value, ms=ms.value)
 
@@ -348,10 +392,13 @@ class Scene:
         g, acc = self._grad_buffer(grad)
         F = C.c_double()
         st = AdjointStats()
+        self.ctx._bind()
         self.ctx.check(self.L.simpler_adjoint_render(self.h, n, n_total or n, seed, offset, C.c_void_p(G.data_ptr()),
                                                      X, Y, Z, C.c_void_p(g.data_ptr()), int(acc), C.byref(F),
                                                      C.byref(st)))
-        return g, F.value, st.as_dict()
+        st = st.as_dict()
+        _warn_dropped(st)
+        return g, F.value, st
 
     def adjoint_rays(self, g_pos, g_dir, g_len, seed=1, offset=0, grad=None):
         """Adjoint of F = sum g_pos . exit_pos + g_dir . exit_dir + g_len * len
@@ -360,10 +407,13 @@ class Scene:
         g, acc = self._grad_buffer(grad)
         F = C.c_double()
         st = AdjointStats()
+        self.ctx._bind()
         self.ctx.check(self.L.simpler_adjoint_rays(self.h, gl.numel(), seed, offset, C.c_void_p(gp.data_ptr()),
                                                    C.c_void_p(gd.data_ptr()), C.c_void_p(gl.data_ptr()),
                                                    C.c_void_p(g.data_ptr()), int(acc), C.byref(F), C.byref(st)))
-        return g, F.value, st.as_dict()
+        st = st.as_dict()
+        _warn_dropped(st)
+        return g, F.value, st
 
     def field(self, pts, scaling=1.0, tau=0.0):
         """n and grad n at points (CUDA tensor (N, 3) float32)."""
@@ -371,6 +421,7 @@ class Scene:
         pts = pts.contiguous().to(torch.float32)
         n = torch.empty((pts.shape[0],), dtype=torch.float32, device=pts.device)
         g = torch.empty((pts.shape[0], 3), dtype=torch.float32, device=pts.device)
+        self.ctx._bind()
         self.ctx.check(self.L.simpler_field_eval(self.h, pts.shape[0], C.c_void_p(pts.data_ptr()), scaling, tau,
                                                  C.c_void_p(n.data_ptr()), C.c_void_p(g.data_ptr())))
         return n, g
@@ -454,22 +505,28 @@ class Batch:
             grads = torch.empty((self.n_cfg, npar), dtype=torch.float32, device=G.device)
         F = (C.c_double * self.n_cfg)()
         st = AdjointStats()
+        self.ctx._bind()
         self.ctx.check(self.L.simpler_batch_adjoint(self.h, C.c_void_p(G.data_ptr()), C.c_void_p(grads.data_ptr()),
                                                     int(acc), F, C.byref(st)))
-        return grads, np.array(F[:]), st.as_dict()
+        st = st.as_dict()
+        _warn_dropped(st)
+        return grads, np.array(F[:]), st
 
     def render(self, accumulate=False, stats=True):
         st = Stats()
+        self.ctx._bind()
         self.ctx.check(self.L.simpler_batch_render(self.h, C.c_void_p(self.images.data_ptr()), int(accumulate),
                                                    C.byref(st) if stats else None))
         return self.images, (st.as_dict() if stats else None)
 
     def capture(self):
         """Capture zero + render of self.images into a HIP graph."""
+        self.ctx._bind()
         self.ctx.check(self.L.simpler_batch_graph_capture(self.h, C.c_void_p(self.images.data_ptr())))
         self.captured = True
 
     def replay(self):
+        self.ctx._bind()
         self.ctx.check(self.L.simpler_batch_graph_launch(self.h))
         return self.images
 
