@@ -265,6 +265,26 @@ int simpler_trace_rays(simpler_scene *scene, int64_t n_rays, uint64_t seed,
                        float *out_dir_dev, float *out_len_dev,
                        int32_t *out_status_dev);
 
+/* Forward mode (tangents) of the per-ray direct trace of simpler_trace_rays:
+ * the same records plus their tangents along a direction of the field
+ * parameters (theta_dot_dev: n_params floats in the layout of
+ * simpler_adjoint_param_count, or NULL) and of each ray's initial velocity
+ * v0 = n(p0) d0 (v0_dot_dev: 3 floats per ray, or NULL).  Replaces the
+ * reference's er_derivativestep (scene.cpp:351-359, with d2V / d2Path,
+ * scene.h:259-272), which propagates dp/dv0 and dv/dv0 along the leapfrog:
+ * v0_dot = e_i gives their columns.  Face crossings are differentiated with
+ * the implicit crossing time, reflections by reflecting the tangent.
+ * Outputs: tan_pos_dev (3 per ray), tan_dir_dev (3, tangent of the unit
+ * direction), tan_len_dev (1, optical length).  Asynchronous on the ctx
+ * stream.  Together with simpler_adjoint_rays it gives the exact
+ * dot-product identity <g, J theta_dot> = <J^T g, theta_dot>. */
+int simpler_jvp_rays(simpler_scene *scene, int64_t n_rays, uint64_t seed,
+                     uint64_t photon_offset, const float *theta_dot_dev,
+                     const float *v0_dot_dev, float *out_pos_dev,
+                     float *out_dir_dev, float *out_len_dev,
+                     int32_t *out_status_dev, float *tan_pos_dev,
+                     float *tan_dir_dev, float *tan_len_dev);
+
 /* Refractive index n and its gradient at n_pts points (device arrays:
  * pts 3*n_pts, out_n n_pts, out_grad 3*n_pts) for the per-photon amplitude
  * scaling s and duty phase tau -- the field evaluation of the march
@@ -343,7 +363,11 @@ int simpler_batch_adjoint(simpler_batch *batch, const float *dF_dimages_dev,
  *   spline         the N0*N1*N2 coefficients, layout of simpler_scene_set_spline;
  *   point sources  2*n_src + 1 values: amplitudes a_i, phases phi_i, then the
  *                  wavenumber k = 2 pi f_u / speed_u (n_src <= 16);
- *   Bessel         none (the adjoint calls fail).
+ *   Bessel         3 values: the amplitude n_max (with n_clip / n_max held
+ *                  fixed), the wavenumber k = 2 pi f_u / speed_u, and a shift
+ *                  of the duty window [phi_min, phi_max] (the transducer
+ *                  phase); scenes in the reference's component form (general
+ *                  axis off) are accepted for the axis -x, or +x at m = 0.
  * Every photon is re-traced from its seed exactly as by simpler_render; paths
  * are differentiated with their random numbers held fixed (the Fresnel
  * transmission factor and the discrete choices -- time bin, face, reflect or
@@ -391,12 +415,23 @@ int simpler_ctx_set_tuning(simpler_ctx *ctx, int block_size,
 int simpler_ctx_get_tuning(simpler_ctx *ctx, int *block_size,
                            int *steps_per_chunk, int *blocks_per_cu);
 
+/* Splat strategy of the render kernels (an autotuner candidate): 0 = each
+ * lane issues its splat's 4 device atomics (cuda_renderer.cu:457-509 does the
+ * same per thread); 1 = wave-aggregated splats: after each event phase the
+ * wave groups its lanes' pending splats by image plane and base pixel, sums
+ * each group's weights across the wave and issues the group's 4 atomics once.
+ * Strategy 1 is compiled for single-scene Bessel renders in the axis frame
+ * (uz = +z, ux = +x); other renders splat per lane whatever the setting. */
+int simpler_ctx_set_splat(simpler_ctx *ctx, int strategy);
+int simpler_ctx_get_splat(simpler_ctx *ctx, int *strategy);
+
 /* Autotuner (no reference counterpart: the reference hard-codes its launch,
  * cuda_renderer.cu:763-767): times an n_photons render of the scene at the
  * given resolution for every candidate (block size 128/256, steps per chunk
- * 32/64/128, blocks per CU 3/4/5), keeps the fastest on the scene's context
- * and returns it (outputs may be NULL).  The splat has a single strategy
- * (per-tap device atomics), so it is not a candidate.  Synchronous. */
+ * 32/64/128, blocks per CU 3/4/5, and the splat strategy 0/1 where strategy 1
+ * exists for the scene), keeps the fastest on the scene's context and returns
+ * it (outputs may be NULL; the splat strategy is read back with
+ * simpler_ctx_get_splat).  Synchronous. */
 int simpler_autotune(simpler_scene *scene, int64_t n_photons, int x_res,
                      int y_res, int z_res, int *block_size, int *steps_per_chunk,
                      int *blocks_per_cu, float *ms_best);

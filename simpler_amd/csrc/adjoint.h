@@ -367,7 +367,7 @@ __device__ __forceinline__ void sub_rev_start(Lane &L, f3 lp, f3 lv, float mu, f
 //   val = W 2 pi exp(-sigma d) f_HG(din . dts) fore fw,
 //   fore = (refr . c) / (dts . c); fw (Fresnel transmission) is held fixed.
 template <int FIELD>
-__device__ bool nee_adj_start(Lane &L, f3 ps, float Lv, float val, f3 din, f3 dts, float fore) {
+__device__ __forceinline__ bool nee_adj_start(Lane &L, f3 ps, float Lv, float val, f3 din, f3 dts, float fore) {
     AdjState &a = *L.a;
     const DevScene &S = L.S;
     float Wsum;
@@ -398,7 +398,7 @@ __device__ bool nee_adj_start(Lane &L, f3 ps, float Lv, float val, f3 din, f3 dt
 
 // Direct splat (photon.cpp:53-80): val = W exp(-sigma d) fw.
 template <int FIELD>
-__device__ bool direct_adj_start(Lane &L, f3 ps, float Lv, float val) {
+__device__ __forceinline__ bool direct_adj_start(Lane &L, f3 ps, float Lv, float val) {
     AdjState &a = *L.a;
     const DevScene &S = L.S;
     float Wsum;
@@ -418,7 +418,7 @@ __device__ bool direct_adj_start(Lane &L, f3 ps, float Lv, float val) {
 
 // Per-ray objective gp . p + gd . d + gL L at the sensor-face exit.
 template <int FIELD>
-__device__ bool ray_adj_start(Lane &L) {
+__device__ __forceinline__ bool ray_adj_start(Lane &L) {
     AdjState &a = *L.a;
     const RenderArgs &A = L.A;
     const Hot &h = L.h;

@@ -23,7 +23,7 @@ __device__ __forceinline__ void rev_load(Lane &L, const float *base, int idx, f3
 // path at the last such vertex.  mu is minus the adjoint of the flight's
 // distance budget, which moves the vertex along the path.
 template <int FIELD>
-__device__ void main_rev_init(Lane &L) {
+__device__ __forceinline__ void main_rev_init(Lane &L) {
     AdjState &a = *L.a;
     Hot &h = L.h;
     a.rphase = RP_MAIN;
@@ -39,18 +39,29 @@ __device__ void main_rev_init(Lane &L) {
     rev_init(h, a, L.S.h0, cnt0, pat);
 }
 
-// The backward run reached the start of a segment.
+// The start point of a logged segment end (the next segment's start).
+__device__ __forceinline__ f3 rec_point(const float *base, int idx) {
+    const float4 r = *reinterpret_cast<const float4 *>(base + (size_t)idx * kRecFloats);
+    return mk3(r.x, r.y, r.z);
+}
+
+// The backward run reached the start of a segment.  The pending parameter
+// derivatives of the start point are added at the point the forward run
+// started from -- the logged segment end, the NEE vertex or the source --
+// not at the backward run's reconstruction of it, which drifts by rounding
+// (a source on a gap plane could otherwise fall just outside the gap).
 template <int FIELD, int MODE>
-__device__ void rev_end(Lane &L) {
+__device__ __forceinline__ void rev_end(Lane &L) {
     AdjState &a = *L.a;
     Hot &h = L.h;
     const DevScene &S = L.S;
     const FieldP &F = L.v->f;
-    field_vjp<FIELD>(S, F, h.p, h.scaling, h.tau, a.pa, a.pb, a.G);
     f3 lamp, lamv;
     uint32_t cnt0, pat, kind;
     if (a.rphase == RP_MAIN) {
         if (a.rseg == 0) {  // the source: v0 = n(p0) d0 with p0, d0 fixed
+            h.p = L.c.src;
+            field_vjp<FIELD>(S, F, h.p, h.scaling, h.tau, a.pa, a.pb, a.G);
             const float vl = len(h.v);
             field_vjp<FIELD>(S, F, h.p, h.scaling, h.tau, dot(a.lv, h.v) / vl, mk3(0.f, 0.f, 0.f), a.G);
             a.pending = false;
@@ -59,6 +70,7 @@ __device__ void rev_end(Lane &L) {
         }
         const f3 vout = h.v;
         --a.rseg;
+        field_vjp<FIELD>(S, F, rec_point(a.log_main, a.rseg), h.scaling, h.tau, a.pa, a.pb, a.G);
         rev_load<FIELD>(L, a.log_main, a.rseg, lamp, lamv, cnt0, pat, kind);
         if (kind & K_FACE) {  // mirror reflection, then the face-crossing time
             const f3 nrm = face_normal((int)(kind >> 1));
@@ -75,6 +87,7 @@ __device__ void rev_end(Lane &L) {
     }
     if (a.rseg > 0) {  // a face reflection inside the sub-path
         --a.rseg;
+        field_vjp<FIELD>(S, F, rec_point(a.log_sub, a.rseg), h.scaling, h.tau, a.pa, a.pb, a.G);
         rev_load<FIELD>(L, a.log_sub, a.rseg, lamp, lamv, cnt0, pat, kind);
         const f3 nrm = face_normal((int)(kind >> 1));
         a.lv = reflect_vec(a.lv, nrm);
@@ -82,7 +95,11 @@ __device__ void rev_end(Lane &L) {
         rev_init(h, a, S.h0, cnt0, pat);
         return;
     }
-    // sub-path start: v0 = n(p0) omega with omega fixed
+    // sub-path start (the NEE vertex, or the source for the direct trace):
+    // v0 = n(p0) omega with omega fixed
+    h.p = L.c.mp;
+    field_vjp<FIELD>(S, F, h.p, h.scaling, h.tau, a.pa, a.pb, a.G);
+    field_eval<FIELD>(S, F, h.p, h.scaling, h.tau, h.n0, h.g0);
     const float vl = len(h.v);
     const float lw = dot(a.lv, h.v) / vl;
     field_vjp<FIELD>(S, F, h.p, h.scaling, h.tau, lw, mk3(0.f, 0.f, 0.f), a.G);
@@ -123,10 +140,13 @@ template <int FIELD, int MODE, bool BATCH = false>
 __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
     __shared__ Cold cold[kBlock];
     __shared__ uint32_t cnt[C_N];
-    __shared__ float sacc[FIELD == kSources ? (2 * kMaxAdjSources + 1) * kBlock : 1];
+    __shared__ float sacc[FIELD == kSources ? (2 * kMaxAdjSources + 1) * kBlock : (FIELD == kBessel ? 3 * kBlock : 1)];
+    __shared__ float vcell[FIELD == kSpline ? 64 * kBlock : 1];  // spline VJP cell sums
+    if (FIELD == kSpline)
+        for (int t = 0; t < 64; ++t) vcell[t * kBlock + threadIdx.x] = 0.f;
     __shared__ Var vars[BATCH ? kBlock : 1];
     if (threadIdx.x < C_N) cnt[threadIdx.x] = 0u;
-    const int npar = FIELD == kSources ? 2 * S.n_src + 1 : 0;
+    const int npar = FIELD == kSources ? 2 * S.n_src + 1 : (FIELD == kBessel ? 3 : 0);
     for (int j = 0; j < npar; ++j) sacc[j * kBlock + threadIdx.x] = 0.f;
     __syncthreads();
     Var vsingle = var_from(S, A);
@@ -148,7 +168,9 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
     a.G.grid = A.grad;
     a.G.acc = sacc + threadIdx.x;
     a.G.stride = kBlock;
-    Lane L{h, cold[threadIdx.x], &v, S, A, cnt, &a, nullptr};
+    a.G.cell = FIELD == kSpline ? vcell + threadIdx.x : nullptr;
+    a.G.key = -1;
+    Lane L{h, cold[threadIdx.x], &v, S, A, cnt, &a, nullptr, nullptr};
     const int lane = __lane_id();
     while (true) {
         const unsigned long long need = __ballot(h.state == ST_NEED);
@@ -208,6 +230,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
         if (h.state == ST_RMARCH && !r) h.state = ST_REND;
     }
     if (BATCH) adj_flush_cfg(a, A, cur_cfg, npar);
+    if (FIELD == kSpline) spline_cell_flush(S, a.G);
     unsigned long long st = h.steps;
     double fo = a.F;
     unsigned long long ac[A_N] = {a.rsteps, a.nsubrev, a.nmainrev, a.dropped};

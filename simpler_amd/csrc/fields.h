@@ -377,6 +377,13 @@ struct GradSink {
     float *grid;  // spline coefficient gradient (device)
     float *acc;   // source-parameter accumulators (LDS, lane-private)
     int stride;
+    // spline: the lane's sums for the 64 taps of one base cell (LDS, stride
+    // `stride`), added to grid when the lane's base cell changes (a
+    // reverse path spends about ten steps in a cell, so this cuts the
+    // device atomics about tenfold); key = the base tap's linear index, -1
+    // when empty; null for kernels without the buffer
+    float *cell;
+    int key;
 };
 
 template <int FIELD>
@@ -384,19 +391,106 @@ __device__ __forceinline__ void field_eval_h(const DevScene &S, const FieldP &F,
                                              float &n, f3 &g, Sym3 &H);
 template <int FIELD>
 __device__ __forceinline__ void field_vjp(const DevScene &S, const FieldP &F, f3 p, float scaling, float tau,
-                                          float a, f3 b, const GradSink &G);
+                                          float a, f3 b, GradSink &G);
 
-// Bessel fields carry no differentiable parameters here; the adjoint entry
-// points reject them.  Defined so that the templates instantiate.
+// Bessel.  With f = A s J_m(k r) cos(m phi) (A = n_max, s = the photon's
+// duty scaling), r and phi the polar coordinates of p about the transducer
+// axis, rh = grad r (the transverse unit vector) and th = uz x rh = r grad phi:
+//   grad f = A s [k J' C rh - (m J S / r) th]            (C, S = cos, sin m phi)
+//   H f    = A s [a rh rh^T + b th th^T + c (rh th^T + th rh^T)],
+//     a = k^2 J'' C,  b = k J' C / r - m^2 J C / r^2,  c = m J S / r^2 - m k J' S / r
+// (the reference's bessel_HessianRIF, scene.cpp:259-320, is the same
+// expansion for its axis = x frame without the scaling factor; here the
+// Hessian is that of the gradient the march uses).  J'' is the derivative of
+// the table's J' polynomial.  The general-axis frame is used throughout; the
+// adjoint accepts the reference's component form only where it equals it.
+// Parameters (field_vjp): theta = (A, k, tau shift), accumulated per lane in
+// acc[0], acc[stride], acc[2 stride]:
+//   d/dA:   n: s J C,                grad: s [k J' C rh - (m J S / r) th]
+//   d/dk:   n: A s J' r C,           grad: A s [(J' + k r J'') C rh - m J' S th]
+//   d/dtau: n: A s' J C,             grad: A s' [k J' C rh - (m J S / r) th]
+// with s' = ds/dtau = cos tau while the clamp |sin tau| <= n_clip / n_max is
+// inactive (0 when it clips); tau shift moves the whole duty window
+// [phi_min, phi_max].  The amplitude derivative holds n_clip / n_max fixed.
+struct BesselLocal {
+    f3 rh, th;
+    float r, J, dJ, d2J, C, Sn, As;
+    int m;
+};
+
+__device__ __forceinline__ void bessel_tab2(const FieldP &F, float x, float &J, float &dJ, float &d2J) {
+    const int i = min((int)x, F.jtab_n - 1);
+    const float u = x - ((float)i + 0.5f);
+    const float4 *t = F.jtab + 4 * i;
+    const float4 a = t[0], b = t[1], c = t[2], d = t[3];
+    J = a.x + u * (a.y + u * (a.z + u * (a.w + u * (b.x + u * (b.y + u * (b.z + u * b.w))))));
+    dJ = c.x + u * (c.y + u * (c.z + u * (c.w + u * (d.x + u * (d.y + u * (d.z + u * d.w))))));
+    d2J = c.y + u * (2.f * c.z + u * (3.f * c.w + u * (4.f * d.x + u * (5.f * d.y + u * (6.f * d.z + u * (7.f * d.w))))));
+}
+
+// The local quantities at p (general-axis frame; r < eps uses the clamped
+// transverse vector of the gradient, reference scene.cpp:229-234).
+__device__ __forceinline__ BesselLocal bessel_local(const DevScene &S, const FieldP &F, f3 p, float scaling) {
+    BesselLocal b;
+    const bool in_gap = !(p.x > S.egap || p.x < S.sgap);
+    b.As = in_gap ? F.amp * scaling : 0.f;
+    b.m = F.mode;
+    const f3 q = p - S.pu;
+    f3 rho = q - dot(q, S.uz) * S.uz;
+    float r2 = dot(rho, rho);
+    float ir = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
+    b.r = r2 * ir;
+    float c1 = r2 > 0.f ? dot(rho, S.ux) * ir : 1.0f, s1 = r2 > 0.f ? dot(rho, S.uy) * ir : 0.0f;
+    if (b.r < kEps) {
+        rho = kEps * (S.ux + S.uy);
+        b.r = kEps;
+        ir = 1.0f / kEps;
+    }
+    float cm = 1.f, sm = 0.f;
+    for (int k = 0; k < b.m; ++k) {
+        const float cn = cm * c1 - sm * s1;
+        sm = sm * c1 + cm * s1;
+        cm = cn;
+    }
+    b.C = cm;
+    b.Sn = sm;
+    b.rh = ir * rho;
+    b.th = cross(S.uz, b.rh);
+    bessel_tab2(F, F.k_r * b.r, b.J, b.dJ, b.d2J);
+    return b;
+}
+
 template <>
 __device__ __forceinline__ void field_eval_h<kBessel>(const DevScene &S, const FieldP &F, f3 p, float scaling,
                                                       float tau, float &n, f3 &g, Sym3 &H) {
     field_eval<kBessel>(S, F, p, scaling, tau, n, g);
-    H = sym_zero();
+    const BesselLocal b = bessel_local(S, F, p, scaling);
+    const float k = F.k_r, m = (float)b.m, ir = 1.0f / b.r;
+    const float a = b.As * k * k * b.d2J * b.C;
+    const float bb = b.As * (k * b.dJ * b.C * ir - m * m * b.J * b.C * ir * ir);
+    const float c = b.As * (m * b.J * b.Sn * ir * ir - m * k * b.dJ * b.Sn * ir);
+    const f3 R = b.rh, T = b.th;
+    H.xx = a * R.x * R.x + bb * T.x * T.x + 2.f * c * R.x * T.x;
+    H.yy = a * R.y * R.y + bb * T.y * T.y + 2.f * c * R.y * T.y;
+    H.zz = a * R.z * R.z + bb * T.z * T.z + 2.f * c * R.z * T.z;
+    H.xy = a * R.x * R.y + bb * T.x * T.y + c * (R.x * T.y + T.x * R.y);
+    H.xz = a * R.x * R.z + bb * T.x * T.z + c * (R.x * T.z + T.x * R.z);
+    H.yz = a * R.y * R.z + bb * T.y * T.z + c * (R.y * T.z + T.y * R.z);
 }
+
 template <>
-__device__ __forceinline__ void field_vjp<kBessel>(const DevScene &, const FieldP &, f3, float, float, float, f3,
-                                                   const GradSink &) {}
+__device__ __forceinline__ void field_vjp<kBessel>(const DevScene &S, const FieldP &F, f3 p, float scaling, float tau,
+                                                   float a, f3 bv, GradSink &G) {
+    if (p.x > S.egap || p.x < S.sgap) return;
+    const BesselLocal b = bessel_local(S, F, p, scaling);
+    const float k = F.k_r, m = (float)b.m, ir = 1.0f / b.r;
+    const float br = dot(bv, b.rh), bt = dot(bv, b.th);
+    const float unit = a * b.J * b.C + k * b.dJ * b.C * br - m * b.J * b.Sn * ir * bt;  // per unit of A s
+    const float dsdt = fabsf(sinf(tau)) < S.n_max_scaling ? cosf(tau) : 0.f;
+    G.acc[0] += scaling * unit;
+    G.acc[G.stride] += b.As * (a * b.dJ * b.r * b.C + (b.dJ + k * b.r * b.d2J) * b.C * br - m * b.dJ * b.Sn * bt);
+    G.acc[2 * G.stride] += F.amp * dsdt * unit;
+}
 
 // Point sources.  Per source, with r = |p - x_i|, psi_c = c k r + phi_i - t:
 //   f  = sum_c w_c cos(psi_c) / r
@@ -455,7 +549,7 @@ __device__ __forceinline__ void field_eval_h<kSources>(const DevScene &S, const 
 // d/dk:     n: -a sum w c sin, grad: -a k sum w c^2 cos rhat
 template <>
 __device__ __forceinline__ void field_vjp<kSources>(const DevScene &S, const FieldP &F, f3 p, float, float tau,
-                                                    float a, f3 b, const GradSink &G) {
+                                                    float a, f3 b, GradSink &G) {
     if (p.x > S.egap || p.x < S.sgap) return;
     const float tshift = S.time_coupling * tau;
     const int N = S.n_src;
@@ -587,11 +681,32 @@ __device__ __forceinline__ void field_eval_h<kSpline>(const DevScene &S, const F
     H = Sym3{vxx * rx * rx, vyy * ry * ry, vzz * rz * rz, vxy * rx * ry, vxz * rx * rz, vyz * ry * rz};
 }
 
+// Adds the lane's cell sums to the coefficient gradient and empties them.
+__device__ __forceinline__ void spline_cell_flush(const DevScene &S, GradSink &G) {
+    if (G.key < 0) return;
+    const long NX = S.sN[0], NXY = (long)S.sN[0] * S.sN[1];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            float *row = G.grid + G.key + c * NXY + b * NX;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float *slot = G.cell + (c * 16 + b * 4 + q) * G.stride;
+                unsafeAtomicAdd(row + q, *slot);
+                *slot = 0.f;
+            }
+        }
+    G.key = -1;
+}
+
 // coefficient (x, y, z) receives a wx wy wz + bx rx dwx wy wz + by ry wx dwy wz
-// + bz rz wx wy dwz  (mirror-wrapped taps may coincide: atomics add them).
+// + bz rz wx wy dwz.  Taps inside the grid go to the lane's cell sums (see
+// GradSink); near the faces, where mirror-wrapped taps may coincide, device
+// atomics add them directly.
 template <>
 __device__ __forceinline__ void field_vjp<kSpline>(const DevScene &S, const FieldP &, f3 p, float, float, float a,
-                                                   f3 b, const GradSink &G) {
+                                                   f3 b, GradSink &G) {
     if (p.x > S.egap || p.x < S.sgap) return;
     const float gx = (p.x - S.sxmin[0]) * S.sxres[0];
     const float gy = (p.y - S.sxmin[1]) * S.sxres[1];
@@ -604,6 +719,24 @@ __device__ __forceinline__ void field_vjp<kSpline>(const DevScene &S, const Fiel
     const int ix = (int)fx - 1, iy = (int)fy - 1, iz = (int)fz - 1;
     const int NX = S.sN[0], NY = S.sN[1], NZ = S.sN[2];
     const float bx = b.x * S.sxres[0], by = b.y * S.sxres[1], bz = b.z * S.sxres[2];
+    if (G.cell && (unsigned)ix <= (unsigned)(NX - 4) && (unsigned)iy <= (unsigned)(NY - 4) &&
+        (unsigned)iz <= (unsigned)(NZ - 4)) {
+        const int key = (iz * NY + iy) * NX + ix;
+        if (key != G.key) {
+            spline_cell_flush(S, G);
+            G.key = key;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int bq = 0; bq < 4; ++bq) {
+                const float Pw = (a * wy[bq] + by * dwy[bq]) * wz[c] + bz * wy[bq] * dwz[c];
+                const float Pd = bx * wy[bq] * wz[c];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) G.cell[(c * 16 + bq * 4 + q) * G.stride] += Pw * wx[q] + Pd * dwx[q];
+            }
+        return;
+    }
     int xi[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) xi[q] = mirror_idx(ix + q, NX);
@@ -622,4 +755,83 @@ __device__ __forceinline__ void field_vjp<kSpline>(const DevScene &S, const Fiel
     }
 }
 
+
+// ====================================================================
+// Tangent of the field along a parameter direction theta_dot (the forward
+// mode of field_vjp, same parameter layout): nt = sum_j theta_dot_j dn /
+// dtheta_j and gt = sum_j theta_dot_j d(grad n) / dtheta_j at p.
+template <int FIELD>
+__device__ __forceinline__ void field_jvp(const DevScene &S, const FieldP &F, f3 p, float scaling, float tau,
+                                          const float *td, float &nt, f3 &gt);
+
+// spline: the field is linear in the coefficients, so its tangent is the
+// spline of the coefficient direction
+template <>
+__device__ __forceinline__ void field_jvp<kSpline>(const DevScene &S, const FieldP &F, f3 p, float, float,
+                                                   const float *td, float &nt, f3 &gt) {
+    nt = 0.f;
+    gt = mk3(0.f, 0.f, 0.f);
+    if (p.x > S.egap || p.x < S.sgap) return;
+    DevScene T = S;
+    T.spline = td;
+    T.spline_pad = nullptr;
+    T.spline_ref = 0.f;
+    field_eval<kSpline>(T, F, p, 0.f, 0.f, nt, gt);
+}
+
+template <>
+__device__ __forceinline__ void field_jvp<kSources>(const DevScene &S, const FieldP &F, f3 p, float, float tau,
+                                                    const float *td, float &nt, f3 &gt) {
+    nt = 0.f;
+    gt = mk3(0.f, 0.f, 0.f);
+    if (p.x > S.egap || p.x < S.sgap) return;
+    const float tshift = S.time_coupling * tau;
+    const int N = S.n_src;
+    const float kd = td[2 * N];
+    for (int i = 0; i < N; ++i) {
+        const float4 s = F.src[i];
+        const f3 dv = mk3(p.x - s.x, p.y - s.y, p.z - s.z);
+        const float r2 = dot(dv, dv);
+        const float ir = rsqrtf(r2);
+        const float r = r2 * ir;
+        const float ph0 = F.phase[i] - tshift;
+        float vs = 0.f, gs = 0.f, ss = 0.f, kc = 0.f, cs1 = 0.f, c2c = 0.f;
+        for (int c = 1; c <= S.n_coeff; ++c) {
+            const float fc = (float)c;
+            const float krc = F.k_r * r * fc;
+            float sn, cs;
+            __sincosf(krc + ph0, &sn, &cs);
+            const float w = S.coeff_w[c - 1];
+            vs += w * cs;
+            gs += w * (cs + krc * sn);
+            ss += w * sn;
+            kc += w * krc * cs;
+            cs1 += w * fc * sn;
+            c2c += w * fc * fc * cs;
+        }
+        const float ad = td[i], pd = td[N + i];
+        // n-part and the radial factor of the gradient part (times rhat)
+        nt += ad * vs * ir + pd * s.w * (-ss * ir) + kd * s.w * (-cs1);
+        const float radial = ad * (-gs * ir * ir) + pd * s.w * (ss - kc) * ir * ir + kd * s.w * (-F.k_r * c2c);
+        gt = gt + (radial * ir) * dv;
+    }
+}
+
+template <>
+__device__ __forceinline__ void field_jvp<kBessel>(const DevScene &S, const FieldP &F, f3 p, float scaling, float tau,
+                                                   const float *td, float &nt, f3 &gt) {
+    nt = 0.f;
+    gt = mk3(0.f, 0.f, 0.f);
+    if (p.x > S.egap || p.x < S.sgap) return;
+    const BesselLocal b = bessel_local(S, F, p, scaling);
+    const float k = F.k_r, m = (float)b.m, ir = 1.0f / b.r;
+    const float dsdt = fabsf(sinf(tau)) < S.n_max_scaling ? cosf(tau) : 0.f;
+    // per unit of A s: n = J C, grad = k J' C rh - (m J S / r) th
+    const float cA = td[0] * scaling + td[2] * F.amp * dsdt;
+    const float ck = td[1] * b.As;
+    nt = cA * b.J * b.C + ck * b.dJ * b.r * b.C;
+    const float gr = cA * k * b.dJ * b.C + ck * (b.dJ + k * b.r * b.d2J) * b.C;
+    const float gtn = -cA * m * b.J * b.Sn * ir - ck * m * b.dJ * b.Sn;
+    gt = gr * b.rh + gtn * b.th;
+}
 }  // namespace simpler

@@ -39,8 +39,9 @@ enum { PH_DIRECT = 0, PH_NEE = 1, PH_FLIGHT = 2 };
 enum { ST_MARCH = 0, ST_SEG_END = 1, ST_NEED = 2, ST_INIT = 3, ST_DONE = 4, ST_RMARCH = 5, ST_REND = 6, ST_MREV = 7 };
 // kernel modes: image render, per-ray record, adjoint of an image objective,
 // adjoint of a per-ray objective
-enum { M_RENDER = 0, M_RECORD = 1, M_ADJ = 2, M_ADJRAYS = 3 };
-__host__ __device__ constexpr bool per_ray(int m) { return m == M_RECORD || m == M_ADJRAYS; }
+// adjoint of a per-ray objective, per-ray record with tangents (forward mode)
+enum { M_RENDER = 0, M_RECORD = 1, M_ADJ = 2, M_ADJRAYS = 3, M_JVP = 4 };
+__host__ __device__ constexpr bool per_ray(int m) { return m == M_RECORD || m == M_ADJRAYS || m == M_JVP; }
 __host__ __device__ constexpr bool is_adj(int m) { return m == M_ADJ || m == M_ADJRAYS; }
 enum { C_PHOT = 0, C_STEPS, C_SCAT, C_NEE, C_DIRECT, C_FACE, C_SEG, C_ABORT, C_N };
 constexpr uint32_t kMaxFaceEvents = 1u << 20;  // per march; see the abort counter
@@ -75,6 +76,17 @@ struct RenderArgs {
     int main_cap, sub_cap;
     const float *up_p, *up_d, *up_L;
     unsigned long long *acnt;
+    // forward mode: parameter direction, optional per-ray initial-velocity
+    // directions, and the tangents of the recorded exit state
+    const float *theta_dot, *v0_dot;
+    float *tan_p, *tan_d, *tan_L;
+};
+
+// Tangent of the photon state along a direction (forward mode): position,
+// total velocity, n and grad n at the position, optical length.
+struct Tan {
+    f3 p, v, g0;
+    float n0, L;
 };
 
 // Hot photon state: what the march loop reads and writes every step; kept in
@@ -98,6 +110,7 @@ struct Cold {
     int phase, depth, first_flight;
     float acc, large, mL, total_dist;
     f3 mp, mv;  // main-path position / direction saved across NEE and direct
+    f3 src;     // the photon's source point (the adjoint's exact main-path start)
 };
 
 // Per-configuration photon parameters: uniform in a single-scene launch, one
@@ -127,6 +140,15 @@ __device__ __forceinline__ Var var_from(const DevScene &S, const RenderArgs &A) 
     return v;
 }
 
+// A splat waiting for the wave's aggregated atomics (wave-aggregated splat
+// strategy): its image plane, base tap and the 4 bilinear weights.
+struct PendSplat {
+    float *plane;
+    int ix, iy;
+    float w[4];
+    bool valid;
+};
+
 struct Lane {
     Hot &h;
     Cold &c;
@@ -136,6 +158,8 @@ struct Lane {
     uint32_t *cnt;          // block-level event counters in LDS
     struct AdjState *a;     // adjoint state (adjoint kernels only)
     float4 *mc;             // n, grad n at the saved main-path point (LDS; render kernels)
+    PendSplat *ps;          // wave-aggregated splats: the lane's pending splat (else null)
+    Tan *t;                 // forward-mode tangents (M_JVP kernels only)
 };
 
 __device__ __forceinline__ void count(Lane &L, int k) { atomicAdd(L.cnt + k, 1u); }
@@ -303,6 +327,93 @@ __device__ __forceinline__ bool march_step(Hot &h, uint32_t &steps_left, const D
     return false;
 }
 
+// march_step with the tangents of the step (forward mode; the reference's
+// er_derivativestep, scene.cpp:351-359 with d2V / d2Path, scene.h:259-272,
+// generalised from d(.)/dv0 to any direction of the initial velocity and
+// the field parameters).  The primal is march_step's; the tangent of the
+// leapfrog is
+//   vh' = v' + h/2 g0',  q' = p' + (h/n) vh' - (h/n) (n0'/n) vh,
+//   n1' = g1 . q' + dn/dtheta . td,  g1' = H1 q' + dg/dtheta . td,
+//   v1' = vh' + h/2 g1',  L' += h n1' (accepted steps),
+// with n, grad n, H evaluated where the primal evaluates them.
+template <int FIELD>
+__device__ __forceinline__ bool march_step_jvp(Hot &h, Tan &t, uint32_t &steps_left, const DevScene &S,
+                                               const FieldP &F, float Lcut, const float *td) {
+    const float hs = h.h;
+    const float hh = 0.5f * hs;
+    const f3 dvh = h.dv + hh * h.g0;
+    const f3 vh = h.v + dvh;
+    const float hn = hs * __builtin_amdgcn_rcpf(h.n0);
+    float px = h.p.x, py = h.p.y, pz = h.p.z;
+    float lx = h.pl.x, ly = h.pl.y, lz = h.pl.z;
+    comp_add(px, lx, hn * vh.x);
+    comp_add(py, ly, hn * vh.y);
+    comp_add(pz, lz, hn * vh.z);
+    const f3 p1 = mk3(px, py, pz);
+    float n1, nh;
+    f3 g1, gh;
+    Sym3 H1;
+    field_eval<FIELD>(S, F, p1, h.scaling, h.tau, n1, g1);
+    field_eval_h<FIELD>(S, F, p1, h.scaling, h.tau, nh, gh, H1);
+    float nth = 0.f;
+    f3 gth = mk3(0.f, 0.f, 0.f);
+    if (td) field_jvp<FIELD>(S, F, p1, h.scaling, h.tau, td, nth, gth);
+    const f3 tvh = t.v + hh * t.g0;
+    const f3 tp1 = t.p + hn * tvh - (hn * t.n0 / h.n0) * vh;
+    const float tn1 = dot(g1, tp1) + nth;
+    const f3 tg1 = symmul(H1, tp1) + gth;
+    const f3 tv1 = tvh + hh * tg1;
+    const f3 dv1 = dvh + hh * g1;
+    const bool reject = !inside_block(S, p1) || (h.travelled + hs) > h.seg_dist;
+    const bool stop = reject && (--h.prec < 0);
+    if (!reject || stop) {
+        h.p = p1;
+        h.pl = mk3(lx, ly, lz);
+        h.dv = dv1;
+        h.n0 = n1;
+        h.g0 = g1;
+        t.p = tp1;
+        t.v = tv1;
+        t.n0 = tn1;
+        t.g0 = tg1;
+    }
+    if (stop) {
+        end_segment(h);
+        return true;
+    }
+    if (reject) {
+        h.h = div10(hs);
+        steps_left = 10;
+        return false;
+    }
+    h.travelled += hs;
+    comp_add(h.L, h.Ll, hs * n1);
+    t.L += hs * tn1;
+    if (h.L > Lcut) {
+        h.dead = true;
+        end_segment(h);
+        return true;
+    }
+    if (--steps_left == 0) {
+        end_segment(h);
+        return true;
+    }
+    return false;
+}
+
+// The implicit crossing time of a segment that ends on the face with
+// outward normal nrm: a tangent moves the crossing by ds = -nrm.p' / nrm.u
+// (u = v / n), and the end state along the path by (u, grad n, n) ds.
+__device__ __forceinline__ void tan_face(Tan &t, const Hot &h, f3 nrm) {
+    const f3 u = h.v / h.n0;
+    const float nu = dot(nrm, u);
+    if (fabsf(nu) < 1e-20f) return;
+    const float ds = -dot(nrm, t.p) / nu;
+    t.p = t.p + ds * u;
+    t.v = t.v + ds * h.g0;
+    t.L += ds * h.n0;
+}
+
 // scene.cpp:655-676: outward normal of the nearest face (strict <, order
 // L0 R0 L1 R1 L2 R2)
 __device__ __forceinline__ f3 nearest_face(const DevScene &S, f3 p) {
@@ -383,9 +494,10 @@ __device__ __forceinline__ float hg_f(float g, f3 in, f3 out) {
 }
 
 // scene.cpp:782-829 -> image.h:320-324, with the 4 bilinear taps as
-// device-wide float atomics
+// device-wide float atomics, issued by the lane (ps null) or left in ps for
+// the wave's aggregated splat (wave_splat)
 __device__ __forceinline__ void splat(const DevScene &S, const RenderArgs &A, float *image, f3 p, float L,
-                                      int depth, float val) {
+                                      int depth, float val, PendSplat *ps = nullptr) {
     float x = dot(S.cam_h, p) - S.cam_o.y;
     float y = dot(S.cam_v, p) - S.cam_o.z;
     if (!(S.nogate || (L > S.pl0 && L < S.pl1))) return;
@@ -409,12 +521,55 @@ __device__ __forceinline__ void splat(const DevScene &S, const RenderArgs &A, fl
     const float fx = x - fx0, fy = y - fy0;
     float *plane = image + (size_t)iz * A.X * A.Y;
     const float w[4] = {val * (1.f - fx) * (1.f - fy), val * fx * (1.f - fy), val * (1.f - fx) * fy, val * fx * fy};
+    if (ps) {
+        ps->plane = plane;
+        ps->ix = ix;
+        ps->iy = iy;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) ps->w[k] = w[k];
+        ps->valid = true;
+        return;
+    }
     const int tx[4] = {ix, ix + 1, ix, ix + 1};
     const int ty[4] = {iy, iy, iy + 1, iy + 1};
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         if (tx[k] >= 0 && tx[k] < A.X && ty[k] >= 0 && ty[k] < A.Y)
             unsafeAtomicAdd(plane + (size_t)ty[k] * A.X + tx[k], w[k]);
+}
+
+// Wave-aggregated splat: the pending splats of the wave's lanes, grouped by
+// image plane and base tap; each group's weights are summed across the wave
+// and its leader issues the 4 atomics once (a group of one splats directly).
+// Called by every lane of the wave (converged).
+__device__ __forceinline__ void wave_splat(PendSplat &ps, const RenderArgs &A) {
+    const int lane = __lane_id();
+    unsigned long long pending = __ballot(ps.valid);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const unsigned long long kp = __shfl((unsigned long long)(uintptr_t)ps.plane, leader);
+        const int kix = __shfl(ps.ix, leader), kiy = __shfl(ps.iy, leader);
+        const bool mine = ps.valid && (unsigned long long)(uintptr_t)ps.plane == kp && ps.ix == kix && ps.iy == kiy;
+        const unsigned long long match = __ballot(mine);
+        float w[4] = {mine ? ps.w[0] : 0.f, mine ? ps.w[1] : 0.f, mine ? ps.w[2] : 0.f, mine ? ps.w[3] : 0.f};
+        if (__popcll(match) > 1) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) w[k] += __shfl_xor(w[k], off);
+        }
+        if (lane == leader) {
+            float *plane = ps.plane;
+            const int tx[4] = {kix, kix + 1, kix, kix + 1};
+            const int ty[4] = {kiy, kiy, kiy + 1, kiy + 1};
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (tx[k] >= 0 && tx[k] < A.X && ty[k] >= 0 && ty[k] < A.Y)
+                    unsafeAtomicAdd(plane + (size_t)ty[k] * A.X + tx[k], w[k]);
+        }
+        if (mine) ps.valid = false;
+        pending &= ~match;
+    }
 }
 
 // lens.cpp:69-97
@@ -467,7 +622,7 @@ __device__ __forceinline__ void flight_start(Lane &L, float dist) {
 }
 
 // photon.cpp:19-38
-__device__ void scatter_once(Lane &L) {
+__device__ __forceinline__ void scatter_once(Lane &L) {
     const float a = L.v->albedo;
     if (!((a > 0.f) && ((a >= 1.f) || (draw_u(L) < a)))) {
         photon_end(L);
@@ -485,7 +640,7 @@ __device__ void scatter_once(Lane &L) {
 }
 
 // photon.cpp:97-104: albedo test and first free flight
-__device__ void scatter_start(Lane &L) {
+__device__ __forceinline__ void scatter_start(Lane &L) {
     const float a = L.v->albedo;
     if (!((a > 0.f) && ((a >= 1.f) || (draw_u(L) < a)))) {
         photon_end(L);
@@ -510,7 +665,7 @@ __device__ __forceinline__ void restore_main(Lane &L, float Lval) {
 namespace simpler {
 
 // photon.cpp:111-116 loop head + scene.cpp:866-893 (NEE start)
-__device__ void vertex(Lane &L) {
+__device__ __forceinline__ void vertex(Lane &L) {
     const DevScene &S = L.S;
     Hot &h = L.h;
     Cold &c = L.c;
@@ -548,7 +703,7 @@ __device__ void vertex(Lane &L) {
 // After the direct trace (and its adjoint): back to the main path
 // (photon.cpp:331-333).
 template <int FIELD>
-__device__ void direct_continue(Lane &L, int res) {
+__device__ __forceinline__ void direct_continue(Lane &L, int res) {
     const DevScene &S = L.S;
     if (res == -2 && S.direct_leak) {  // the leaked length is already dead
         photon_end(L);
@@ -560,14 +715,14 @@ __device__ void direct_continue(Lane &L, int res) {
 }
 
 template <int FIELD>
-__device__ void nee_continue(Lane &L) {
+__device__ __forceinline__ void nee_continue(Lane &L) {
     restore_main(L, L.c.mL);
     restore_field<FIELD>(L);
     scatter_once(L);
 }
 
 template <int FIELD, int MODE>
-__device__ void direct_done(Lane &L, int res) {
+__device__ __forceinline__ void direct_done(Lane &L, int res) {
     const DevScene &S = L.S;
     Hot &h = L.h;
     Cold &c = L.c;
@@ -576,7 +731,7 @@ __device__ void direct_done(Lane &L, int res) {
         photon_end(L);
         return;
     }
-    if constexpr (MODE == M_RECORD) {
+    if constexpr (MODE == M_RECORD || MODE == M_JVP) {
         const RenderArgs &A = L.A;
         const size_t i = (size_t)((((uint64_t)c.pid_hi << 32) | c.pid_lo) - A.offset);
         const f3 d = normalized(h.v);
@@ -588,6 +743,17 @@ __device__ void direct_done(Lane &L, int res) {
         A.rec_d[3 * i + 2] = d.z;
         A.rec_L[i] = h.L;
         A.rec_s[i] = res;
+        if constexpr (MODE == M_JVP) {
+            const Tan &t = *L.t;
+            const f3 td = (t.v - dot(d, t.v) * d) / len(h.v);  // tangent of v / |v|
+            A.tan_p[3 * i] = t.p.x;
+            A.tan_p[3 * i + 1] = t.p.y;
+            A.tan_p[3 * i + 2] = t.p.z;
+            A.tan_d[3 * i] = td.x;
+            A.tan_d[3 * i + 1] = td.y;
+            A.tan_d[3 * i + 2] = td.z;
+            A.tan_L[i] = t.L;
+        }
         photon_end(L);
         return;
     }
@@ -601,7 +767,7 @@ __device__ void direct_done(Lane &L, int res) {
             if constexpr (MODE == M_ADJ) {
                 if (direct_adj_start<FIELD>(L, p, Lv, val)) return;
             } else {
-                splat(S, L.A, L.v->image, p, Lv, 0, val);
+                splat(S, L.A, L.v->image, p, Lv, 0, val, L.ps);
             }
         }
     }
@@ -609,7 +775,7 @@ __device__ void direct_done(Lane &L, int res) {
 }
 
 template <int FIELD, int MODE>
-__device__ void nee_done(Lane &L, int res) {
+__device__ __forceinline__ void nee_done(Lane &L, int res) {
     const DevScene &S = L.S;
     Hot &h = L.h;
     Cold &c = L.c;
@@ -624,7 +790,7 @@ __device__ void nee_done(Lane &L, int res) {
             if constexpr (MODE == M_ADJ) {
                 if (nee_adj_start<FIELD>(L, p, Lv, val, din, dts, fore)) return;
             } else {
-                splat(S, L.A, L.v->image, p, Lv, c.depth, val);
+                splat(S, L.A, L.v->image, p, Lv, c.depth, val, L.ps);
             }
         }
     }
@@ -634,7 +800,7 @@ __device__ void nee_done(Lane &L, int res) {
 // End of a march segment: the face / sensor logic of movePhoton
 // (scene.cpp:719-779) and movePhotonTillSensor (scene.cpp:634-700).
 template <int FIELD, int MODE>
-__device__ void seg_end(Lane &L) {
+__device__ __forceinline__ void seg_end(Lane &L) {
     const DevScene &S = L.S;
     Hot &h = L.h;
     Cold &c = L.c;
@@ -683,11 +849,16 @@ __device__ void seg_end(Lane &L) {
             const f3 nrm = nearest_face(S, h.p);
             if (fabsf(S.cam_dir.x - nrm.x) < kEps && fabsf(S.cam_dir.y - nrm.y) < kEps &&
                 fabsf(S.cam_dir.z - nrm.z) < kEps) {
+                if constexpr (MODE == M_JVP) tan_face(*L.t, h, S.cam_dir);
                 res = 1;
             } else {
                 const f3 d1 = bsdf_sample(L, h.v, nrm);  // unnormalised v (scene.cpp:686)
                 if (dot(d1, nrm) < 0.f) {
                     if constexpr (is_adj(MODE)) adj_log_sub(L, nrm);
+                    if constexpr (MODE == M_JVP) {  // the face crossing, then the mirror reflection
+                        tan_face(*L.t, h, nrm);
+                        L.t->v = L.t->v - (2.f * dot(L.t->v, nrm)) * nrm;
+                    }
                     h.v = d1;
                     start_segment(L, c.large);
                     return;
@@ -721,7 +892,7 @@ __device__ __forceinline__ int cdf_sample(const DevScene &S, float u) {
 // photon.cpp:312-333 up to the direct trace: genRay (scene.cpp:107-137,
 // lens.cpp:19-33), duty-cycle phase, v = n d.
 template <int FIELD, int MODE>
-__device__ void photon_init(Lane &L) {
+__device__ __forceinline__ void photon_init(Lane &L) {
     const DevScene &S = L.S;
     Hot &h = L.h;
     Cold &c = L.c;
@@ -751,6 +922,7 @@ __device__ void photon_init(Lane &L) {
     pos = pos + dl * dir;
     total += dl;
     if (S.elens_active && !lens_deflect(S.elens_ap2, S.elens_f, pos, dir, total)) {
+        if constexpr (MODE == M_JVP) *L.t = Tan{mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f), 0.f, 0.f};
         if (per_ray(MODE)) {
             h.p = pos;
             h.v = dir;
@@ -765,9 +937,25 @@ __device__ void photon_init(Lane &L) {
     h.tau = L.v->phi_min + L.v->phi_range * u[5];
     h.scaling = fmaxf(fminf(sinf(h.tau), L.v->n_max_scaling), -L.v->n_max_scaling);
     h.p = pos;
+    if constexpr (is_adj(MODE)) c.src = pos;
     refresh_field<FIELD>(L);
     h.v = h.n0 * dir;
     h.L = total;
+    if constexpr (MODE == M_JVP) {  // v0 = n(p0) d0 with p0, d0 fixed (plus the ray's own v0 direction)
+        Tan &t = *L.t;
+        float nth = 0.f;
+        f3 gth = mk3(0.f, 0.f, 0.f);
+        if (L.A.theta_dot) field_jvp<FIELD>(S, L.v->f, h.p, h.scaling, h.tau, L.A.theta_dot, nth, gth);
+        t.p = mk3(0.f, 0.f, 0.f);
+        t.n0 = nth;
+        t.g0 = gth;
+        t.v = nth * dir;
+        t.L = 0.f;
+        if (L.A.v0_dot) {
+            const size_t i = (size_t)((((uint64_t)c.pid_hi << 32) | c.pid_lo) - L.A.offset);
+            t.v = t.v + mk3(L.A.v0_dot[3 * i], L.A.v0_dot[3 * i + 1], L.A.v0_dot[3 * i + 2]);
+        }
+    }
     if (S.use_direct || per_ray(MODE)) {
         if (L.a) adj_sub_begin(*L.a);
         if (L.mc) *L.mc = make_float4(h.n0, h.g0.x, h.g0.y, h.g0.z);
@@ -808,7 +996,9 @@ namespace simpler {
 // MM: the Bessel order when it is known at launch (single-scene renders of
 // m = 0 or 2), -1 otherwise; GA: the axis form then (1 general, 0 the
 // reference's axis = x form).
-template <int FIELD, int MODE, bool BATCH, bool JLDS = false, int MM = -1, int GA = -1>
+// SPLAT: 0 = every lane issues its splat's atomics, 1 = wave-aggregated
+// splats (wave_splat after the event phase); an autotuner candidate.
+template <int FIELD, int MODE, bool BATCH, bool JLDS = false, int MM = -1, int GA = -1, int SPLAT = 0>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FIELD == kBessel ? 5 : (FIELD == kSpline ? 3 : 1))))
 k_render(DevScene S, RenderArgs A) {
     __shared__ Cold cold[kBlock];
@@ -831,7 +1021,11 @@ k_render(DevScene S, RenderArgs A) {
     h.steps = 0;
     h.state = ST_NEED;
     h.dead = false;
-    Lane L{h, cold[threadIdx.x], vp, S, A, cnt, nullptr, &mcache[threadIdx.x]};
+    PendSplat ps;
+    ps.valid = false;
+    Tan tan;
+    Lane L{h, cold[threadIdx.x], vp, S, A, cnt, nullptr, &mcache[threadIdx.x], SPLAT ? &ps : nullptr,
+           MODE == M_JVP ? &tan : nullptr};
     const int lane = __lane_id();
     unsigned long long wave_steps = 0;  // ER steps of the wave (uniform)
     while (true) {
@@ -869,6 +1063,7 @@ k_render(DevScene S, RenderArgs A) {
         }
         if (h.state == ST_INIT) photon_init<FIELD, MODE>(L);
         if (h.state == ST_SEG_END) seg_end<FIELD, MODE>(L);
+        if (SPLAT) wave_splat(ps, A);
         if (__ballot(h.state != ST_DONE) == 0ull) break;
         if (__ballot(h.state == ST_MARCH) == 0ull) continue;
         FieldP F = L.v->f;
@@ -882,7 +1077,11 @@ k_render(DevScene S, RenderArgs A) {
         uint32_t sl = h.steps_left;
         for (int k = 0; k < A.spc; ++k) {
             wave_steps += __popcll(__ballot(m));  // one ER step per marching lane
-            if (m) m = !march_step<FIELD, false, MM, false, GA>(h, sl, S, F, Lcut);
+            if constexpr (MODE == M_JVP) {
+                if (m) m = !march_step_jvp<FIELD>(h, tan, sl, S, F, Lcut, A.theta_dot);
+            } else {
+                if (m) m = !march_step<FIELD, false, MM, false, GA>(h, sl, S, F, Lcut);
+            }
         }
         h.steps_left = sl;
         if (h.state == ST_MARCH && !m) h.state = ST_SEG_END;
@@ -946,6 +1145,7 @@ struct simpler_ctx {
     unsigned long long *d_scratch = nullptr;  // [0..7] work, [8..15] counters
     int num_cu = 256;
     int block = 256, spc = 32, bpc = 4;
+    int splat = 0;  // splat strategy: 0 per-lane atomics, 1 wave-aggregated
     int live = 0;  // scenes and batches not yet destroyed
     float *d_log = nullptr;  // adjoint segment logs (grown on demand)
     size_t log_bytes = 0;
@@ -1247,17 +1447,22 @@ static void launch_render(simpler_ctx *ctx, int field, const DevScene &S, const 
     switch (field) {
         case SIMPLER_FIELD_BESSEL:
             if (MODE == M_RENDER && S.jtab_n <= kJtabLds && bessel_variant(S) >= 0) {
-                switch (bessel_variant(S)) {
-#define SIMPLER_RENDER_BV(MM, GA)                                                                                  \
-    case 3 * (MM) + (GA):                                                                                          \
-        hipLaunchKernelGGL((k_render<kBessel, M_RENDER, false, true, MM, GA>), grid, block, 0, ctx->cur, S, A); \
+                // the wave-aggregated splat is compiled for the axis-frame
+                // kernels (form 2); the other forms splat per lane
+                const int bv = bessel_variant(S) + (ctx->splat && bessel_variant(S) % 3 == 2 ? 100 : 0);
+                switch (bv) {
+#define SIMPLER_RENDER_BV(MM, GA, SP)                                                                                \
+    case 3 * (MM) + (GA) + 100 * (SP):                                                                               \
+        hipLaunchKernelGGL((k_render<kBessel, M_RENDER, false, true, MM, GA, SP>), grid, block, 0, ctx->cur, S, A); \
         break;
-                    SIMPLER_RENDER_BV(0, 0)
-                    SIMPLER_RENDER_BV(0, 1)
-                    SIMPLER_RENDER_BV(0, 2)
-                    SIMPLER_RENDER_BV(2, 0)
-                    SIMPLER_RENDER_BV(2, 1)
-                    SIMPLER_RENDER_BV(2, 2)
+                    SIMPLER_RENDER_BV(0, 0, 0)
+                    SIMPLER_RENDER_BV(0, 1, 0)
+                    SIMPLER_RENDER_BV(0, 2, 0)
+                    SIMPLER_RENDER_BV(2, 0, 0)
+                    SIMPLER_RENDER_BV(2, 1, 0)
+                    SIMPLER_RENDER_BV(2, 2, 0)
+                    SIMPLER_RENDER_BV(0, 2, 1)
+                    SIMPLER_RENDER_BV(2, 2, 1)
 #undef SIMPLER_RENDER_BV
                 }
             } else if (S.jtab_n <= kJtabLds)
@@ -1419,7 +1624,7 @@ int simpler_ctx_set_tuning(simpler_ctx *ctx, int block_size, int steps_per_chunk
 
 static int render_impl(simpler_scene *sc, int64_t n_photons, int64_t n_total, uint64_t seed, uint64_t offset,
                        float *image, int X, int Y, int Z, int accumulate, bool record, float *rp, float *rd, float *rL,
-                       int32_t *rs);
+                       int32_t *rs, unsigned long long *words = nullptr);
 
 int simpler_autotune(simpler_scene *sc, int64_t n_photons, int X, int Y, int Z, int *block_out, int *spc_out,
                      int *bpc_out, float *ms_out) {
@@ -1438,17 +1643,29 @@ int simpler_autotune(simpler_scene *sc, int64_t n_photons, int X, int Y, int Z, 
         return fail(ctx, SIMPLER_ERR_HIP, "autotune: event creation failed");
     }
     const int blocks[] = {128, 256}, spcs[] = {32, 64, 128}, bpcs[] = {3, 4, 5};
-    int best[3] = {ctx->block, ctx->spc, ctx->bpc};
+    // the splat strategy is a candidate where the aggregated kernel exists
+    // (single-scene Bessel renders in the axis frame)
+    const DevScene S0 = make_dev_scene(sc, 1, 0, false);
+    const bool has_agg = sc->desc.field_kind == SIMPLER_FIELD_BESSEL && S0.jtab_n <= kJtabLds &&
+                         bessel_variant(S0) >= 0 && bessel_variant(S0) % 3 == 2;
+    int best[4] = {ctx->block, ctx->spc, ctx->bpc, 0};
     float best_ms = INFINITY;
     rc = SIMPLER_OK;
     bool warm = false;
+    // launch shapes with per-lane splats, then the aggregated splat at the
+    // best shape (the strategy changes the event phase, not the march)
+    for (int stage = 0; stage < (has_agg ? 2 : 1); ++stage)
     for (int b : blocks)
         for (int spc : spcs)
             for (int bpc : bpcs) {
                 if (rc) break;
+                if (stage == 1 && !(b == best[0] && spc == best[1] && bpc == best[2])) continue;
+                const int sp = stage;
                 ctx->block = b;
                 ctx->spc = spc;
                 ctx->bpc = bpc;
+                ctx->splat = sp;
+                if (stage == 1) warm = false;  // the aggregated kernel's first launch
                 if (!warm) {  // first launch of the kernel: load the code object
                     rc = render_impl(sc, n_photons / 8 + 1, n_photons, 1, 0, img, X, Y, Z, 0, false, nullptr, nullptr,
                                      nullptr, nullptr);
@@ -1466,11 +1683,13 @@ int simpler_autotune(simpler_scene *sc, int64_t n_photons, int X, int Y, int Z, 
                     best[0] = b;
                     best[1] = spc;
                     best[2] = bpc;
+                    best[3] = sp;
                 }
             }
     ctx->block = best[0];
     ctx->spc = best[1];
     ctx->bpc = best[2];
+    ctx->splat = best[3];
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(img);
@@ -1479,6 +1698,19 @@ int simpler_autotune(simpler_scene *sc, int64_t n_photons, int X, int Y, int Z, 
     if (spc_out) *spc_out = best[1];
     if (bpc_out) *bpc_out = best[2];
     if (ms_out) *ms_out = best_ms;
+    return SIMPLER_OK;
+}
+
+int simpler_ctx_set_splat(simpler_ctx *ctx, int strategy) {
+    if (!ctx) return SIMPLER_ERR_INVALID;
+    if (strategy != 0 && strategy != 1) return fail(ctx, SIMPLER_ERR_INVALID, "splat strategy must be 0 or 1");
+    ctx->splat = strategy;
+    return SIMPLER_OK;
+}
+
+int simpler_ctx_get_splat(simpler_ctx *ctx, int *strategy) {
+    if (!ctx || !strategy) return SIMPLER_ERR_INVALID;
+    *strategy = ctx->splat;
     return SIMPLER_OK;
 }
 
@@ -1796,10 +2028,13 @@ int simpler_scene_get_spline(const simpler_scene *sc, float *coeffs_host) {
     return SIMPLER_OK;
 }
 
+// words: the launch's work queue ([0]) and counters ([8..15]); the
+// context's buffer unless the caller brings its own.
 static int render_impl(simpler_scene *sc, int64_t n_photons, int64_t n_total, uint64_t seed, uint64_t offset,
                        float *image, int X, int Y, int Z, int accumulate, bool record, float *rp, float *rd, float *rL,
-                       int32_t *rs) {
+                       int32_t *rs, unsigned long long *words) {
     simpler_ctx *ctx = sc->ctx;
+    if (!words) words = ctx->d_scratch;
     int rc = scene_ready(sc);
     if (rc) return rc;
     if (n_photons < 0) return fail(ctx, SIMPLER_ERR_INVALID, "n_photons must be >= 0");
@@ -1822,15 +2057,15 @@ static int render_impl(simpler_scene *sc, int64_t n_photons, int64_t n_total, ui
     A.Z = Z;
     A.offset = offset;
     A.n = (uint64_t)n_photons;
-    A.work = ctx->d_scratch;
-    A.counters = ctx->d_scratch + 8;
+    A.work = words;
+    A.counters = words + 8;
     A.rec_p = rp;
     A.rec_d = rd;
     A.rec_L = rL;
     A.rec_s = rs;
     A.spc = ctx->spc;
     // zero work queue + counters (a fill kernel: safe under graph capture)
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, ctx->d_scratch, 16);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, words, 16);
     if (!record && !accumulate) {
         const size_t npix = (size_t)X * Y * Z;
         const int blocks = (int)std::min<size_t>((npix + 255) / 256, 8192);
@@ -1846,10 +2081,9 @@ static int render_impl(simpler_scene *sc, int64_t n_photons, int64_t n_total, ui
     return SIMPLER_OK;
 }
 
-int simpler_read_stats(simpler_ctx *ctx, simpler_stats *st) {
-    if (!ctx || !st) return SIMPLER_ERR_INVALID;
+static int read_counters(simpler_ctx *ctx, const unsigned long long *words, simpler_stats *st) {
     unsigned long long c[8];
-    HIP_TRY(ctx, hipMemcpyAsync(c, ctx->d_scratch + 8, sizeof(c), hipMemcpyDeviceToHost, ctx->cur));
+    HIP_TRY(ctx, hipMemcpyAsync(c, words + 8, sizeof(c), hipMemcpyDeviceToHost, ctx->cur));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
     st->photons = c[C_PHOT];
     st->er_steps = c[C_STEPS];
@@ -1860,6 +2094,11 @@ int simpler_read_stats(simpler_ctx *ctx, simpler_stats *st) {
     st->segments = c[C_SEG];
     st->aborted = c[C_ABORT];
     return SIMPLER_OK;
+}
+
+int simpler_read_stats(simpler_ctx *ctx, simpler_stats *st) {
+    if (!ctx || !st) return SIMPLER_ERR_INVALID;
+    return read_counters(ctx, ctx->d_scratch, st);
 }
 
 int simpler_render(simpler_scene *sc, int64_t n_photons, int64_t n_total, uint64_t seed, uint64_t offset,
@@ -1898,6 +2137,42 @@ int simpler_trace_rays(simpler_scene *sc, int64_t n_rays, uint64_t seed, uint64_
     if (!sc) return SIMPLER_ERR_INVALID;
     return render_impl(sc, n_rays, n_rays, seed, offset, nullptr, 1, 1, 1, 0, true, out_pos, out_dir, out_len,
                        out_status);
+}
+
+int simpler_jvp_rays(simpler_scene *sc, int64_t n_rays, uint64_t seed, uint64_t offset, const float *theta_dot,
+                     const float *v0_dot, float *out_pos, float *out_dir, float *out_len, int32_t *out_status,
+                     float *tan_pos, float *tan_dir, float *tan_len) {
+    if (!sc) return SIMPLER_ERR_INVALID;
+    simpler_ctx *ctx = sc->ctx;
+    int rc = scene_ready(sc);
+    if (rc) return rc;
+    if (n_rays < 0) return fail(ctx, SIMPLER_ERR_INVALID, "jvp: n_rays must be >= 0");
+    if (!out_pos || !out_dir || !out_len || !out_status || !tan_pos || !tan_dir || !tan_len)
+        return fail(ctx, SIMPLER_ERR_INVALID, "jvp: null output array");
+    if (sc->desc.sensor_lens_active) return fail(ctx, SIMPLER_ERR_INVALID, "jvp: the sensor lens is not differentiated");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const DevScene S = make_dev_scene(sc, n_rays > 0 ? n_rays : 1, seed, true);
+    RenderArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.X = A.Y = A.Z = 1;
+    A.offset = offset;
+    A.n = (uint64_t)n_rays;
+    A.work = ctx->d_scratch;
+    A.counters = ctx->d_scratch + 8;
+    A.rec_p = out_pos;
+    A.rec_d = out_dir;
+    A.rec_L = out_len;
+    A.rec_s = out_status;
+    A.spc = ctx->spc;
+    A.theta_dot = theta_dot;
+    A.v0_dot = v0_dot;
+    A.tan_p = tan_pos;
+    A.tan_d = tan_dir;
+    A.tan_L = tan_len;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, ctx->d_scratch, 16);
+    if (n_rays > 0) launch_render<M_JVP>(ctx, sc->desc.field_kind, S, A);
+    HIP_TRY(ctx, hipGetLastError());
+    return SIMPLER_OK;
 }
 
 int simpler_field_eval(simpler_scene *sc, int n_pts, const float *pts_dev, float scaling, float tau, float *out_n_dev,
@@ -1953,7 +2228,7 @@ int simpler_adjoint_param_count(const simpler_scene *sc, int64_t *n) {
     switch (sc->desc.field_kind) {
         case SIMPLER_FIELD_SPLINE: *n = (int64_t)sc->sN[0] * sc->sN[1] * sc->sN[2]; return SIMPLER_OK;
         case SIMPLER_FIELD_SOURCES: *n = 2 * (int64_t)sc->n_src + 1; return SIMPLER_OK;
-        default: *n = 0; return fail(sc->ctx, SIMPLER_ERR_INVALID, "adjoint: Bessel fields have no differentiable parameters here");
+        default: *n = 3; return SIMPLER_OK;  // Bessel: amplitude, wavenumber, duty-window shift
     }
 }
 
@@ -1986,6 +2261,16 @@ static int adjoint_impl(simpler_scene *sc, int mode, int64_t n, int64_t n_total,
     if (rc) return rc;
     if (d.field_kind == SIMPLER_FIELD_SOURCES && sc->n_src > kMaxAdjSources)
         return fail(ctx, SIMPLER_ERR_INVALID, "adjoint: at most 16 point sources");
+    if (d.field_kind == SIMPLER_FIELD_BESSEL && !(d.flags & SIMPLER_FLAG_GENERAL_AXIS)) {
+        // the reference's component form equals the general-axis form (which
+        // the adjoint differentiates) for the axis -x, and for m = 0 on +-x
+        const bool mx = d.axis_uz[0] == -1.f && d.axis_uz[1] == 0.f && d.axis_uz[2] == 0.f;
+        const bool px = d.axis_uz[0] == 1.f && d.axis_uz[1] == 0.f && d.axis_uz[2] == 0.f;
+        if (!(mx || (px && d.mode == 0)))
+            return fail(ctx, SIMPLER_ERR_INVALID,
+                        "adjoint: the Bessel component form (general axis off) is differentiated only for the "
+                        "axis -x, or +x at m = 0");
+    }
     if (d.precision > 7) return fail(ctx, SIMPLER_ERR_INVALID, "adjoint: precision must be <= 7");
     if (d.sensor_lens_active) return fail(ctx, SIMPLER_ERR_INVALID, "adjoint: the sensor lens is not differentiated");
     if (!grad) return fail(ctx, SIMPLER_ERR_INVALID, "adjoint: null gradient array");
@@ -2032,17 +2317,21 @@ static int adjoint_impl(simpler_scene *sc, int mode, int64_t n, int64_t n_total,
     }
     if (n > 0) {
         const dim3 grid(blocks), block(kBlock);
-        const bool spline = d.field_kind == SIMPLER_FIELD_SPLINE;
+        const int fk = d.field_kind;
         if (mode == M_ADJ) {
-            if (spline)
+            if (fk == SIMPLER_FIELD_SPLINE)
                 hipLaunchKernelGGL((k_adjoint<kSpline, M_ADJ>), grid, block, 0, ctx->cur, S, A);
-            else
+            else if (fk == SIMPLER_FIELD_SOURCES)
                 hipLaunchKernelGGL((k_adjoint<kSources, M_ADJ>), grid, block, 0, ctx->cur, S, A);
-        } else {
-            if (spline)
-                hipLaunchKernelGGL((k_adjoint<kSpline, M_ADJRAYS>), grid, block, 0, ctx->cur, S, A);
             else
+                hipLaunchKernelGGL((k_adjoint<kBessel, M_ADJ>), grid, block, 0, ctx->cur, S, A);
+        } else {
+            if (fk == SIMPLER_FIELD_SPLINE)
+                hipLaunchKernelGGL((k_adjoint<kSpline, M_ADJRAYS>), grid, block, 0, ctx->cur, S, A);
+            else if (fk == SIMPLER_FIELD_SOURCES)
                 hipLaunchKernelGGL((k_adjoint<kSources, M_ADJRAYS>), grid, block, 0, ctx->cur, S, A);
+            else
+                hipLaunchKernelGGL((k_adjoint<kBessel, M_ADJRAYS>), grid, block, 0, ctx->cur, S, A);
         }
     }
     HIP_TRY(ctx, hipGetLastError());

@@ -64,6 +64,8 @@ def load_library():
     L.simpler_last_error.restype = C.c_char_p
     L.simpler_ctx_set_tuning.argtypes = [P, C.c_int, C.c_int, C.c_int]
     L.simpler_ctx_get_tuning.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.simpler_ctx_set_splat.argtypes = [P, C.c_int]
+    L.simpler_ctx_get_splat.argtypes = [P, C.POINTER(C.c_int)]
     L.simpler_scene_create.argtypes = [P, C.POINTER(SceneDesc), fptr, C.c_int, C.c_int, PP]
     L.simpler_scene_destroy.argtypes = [P]
     L.simpler_scene_update.argtypes = [P, C.POINTER(SceneDesc)]
@@ -78,6 +80,7 @@ def load_library():
     L.simpler_render_host.argtypes = [P, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, fptr, C.c_int, C.c_int,
                                       C.c_int, C.c_int, C.POINTER(Stats)]
     L.simpler_trace_rays.argtypes = [P, C.c_int64, C.c_uint64, C.c_uint64, P, P, P, P]
+    L.simpler_jvp_rays.argtypes = [P, C.c_int64, C.c_uint64, C.c_uint64, P, P, P, P, P, P, P, P, P]
     L.simpler_field_eval.argtypes = [P, C.c_int, P, C.c_float, C.c_float, P, P]
     L.simpler_read_stats.argtypes = [P, C.POINTER(Stats)]
     L.simpler_batch_create.argtypes = [P, C.c_int, C.POINTER(BatchCfg), C.c_int, C.c_int, C.c_int, PP]
@@ -96,9 +99,9 @@ def load_library():
     L.simpler_adjoint_rays.argtypes = [P, C.c_int64, C.c_uint64, C.c_uint64, P, P, P, P, C.c_int,
                                        C.POINTER(C.c_double), C.POINTER(AdjointStats)]
     for name in ("simpler_ctx_create", "simpler_ctx_destroy", "simpler_ctx_set_stream", "simpler_ctx_synchronize",
-                 "simpler_ctx_set_tuning", "simpler_ctx_get_tuning", "simpler_scene_create", "simpler_scene_destroy",
+                 "simpler_ctx_set_tuning", "simpler_ctx_get_tuning", "simpler_ctx_set_splat", "simpler_ctx_get_splat", "simpler_scene_create", "simpler_scene_destroy",
                  "simpler_scene_update", "simpler_scene_get_desc", "simpler_scene_set_sources",
-                 "simpler_scene_set_spline", "simpler_render", "simpler_render_host", "simpler_trace_rays",
+                 "simpler_scene_set_spline", "simpler_render", "simpler_render_host", "simpler_trace_rays", "simpler_jvp_rays",
                  "simpler_field_eval", "simpler_read_stats", "simpler_batch_create", "simpler_batch_render",
                  "simpler_batch_graph_capture", "simpler_batch_graph_launch", "simpler_batch_destroy",
                  "simpler_adjoint_param_count", "simpler_adjoint_render", "simpler_adjoint_rays",
@@ -181,13 +184,16 @@ class Context:
             return cur
         return torch.cuda.ExternalStream(ptr, device=f"cuda:{self.device}")
 
-    def set_tuning(self, block_size=0, steps_per_chunk=0, blocks_per_cu=0):
+    def set_tuning(self, block_size=0, steps_per_chunk=0, blocks_per_cu=0, splat=None):
         self.check(self.L.simpler_ctx_set_tuning(self.h, block_size, steps_per_chunk, blocks_per_cu))
+        if splat is not None:
+            self.check(self.L.simpler_ctx_set_splat(self.h, int(splat)))
 
     def get_tuning(self):
-        b, s, c = C.c_int(), C.c_int(), C.c_int()
+        b, s, c, sp = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         self.check(self.L.simpler_ctx_get_tuning(self.h, C.byref(b), C.byref(s), C.byref(c)))
-        return dict(block_size=b.value, steps_per_chunk=s.value, blocks_per_cu=c.value)
+        self.check(self.L.simpler_ctx_get_splat(self.h, C.byref(sp)))
+        return dict(block_size=b.value, steps_per_chunk=s.value, blocks_per_cu=c.value, splat=sp.value)
 
     def read_stats(self):
         self._bind()
@@ -286,15 +292,17 @@ class Scene:
 
     def params(self):
         """The differentiable field parameters as a float64 vector: spline
-        coefficients, or source amplitudes, phases and the wavenumber
-        k = 2 pi f_u / speed_u."""
+        coefficients; or source amplitudes, phases and the wavenumber
+        k = 2 pi f_u / speed_u; or, for a Bessel transducer, (n_max, k,
+        phi_min) -- the amplitude, the wavenumber and the start of the duty
+        window (its length stays fixed, so this is the transducer phase)."""
+        k = 2.0 * np.pi * float(self.desc.f_u) / float(self.desc.speed_u)
         if self.desc.field_kind == 2:
             return self._spline[3].astype(np.float64)
         if self.desc.field_kind == 1:
             s = self._sources
-            k = 2.0 * np.pi * float(self.desc.f_u) / float(self.desc.speed_u)
             return np.concatenate([s["amp"], s["phase"], [k]]).astype(np.float64)
-        raise SimplerError("Bessel fields have no differentiable parameters")
+        return np.array([float(self.desc.n_max), k, float(self.desc.phi_min)], np.float64)
 
     def set_params(self, vec):
         """Inverse of params()."""
@@ -312,7 +320,15 @@ class Scene:
             d.set(f_u=float(vec[2 * N]) * float(d.speed_u) / (2.0 * np.pi))
             self.update(d)
         else:
-            raise SimplerError("Bessel fields have no differentiable parameters")
+            if vec.size != 3:
+                raise SimplerError("expected 3 Bessel parameters (n_max, k, phi_min)")
+            d = self.desc.copy()
+            ratio = float(d.n_clip) / float(d.n_max)
+            window = float(d.phi_max) - float(d.phi_min)
+            d.set(n_max=float(vec[0]), n_clip=float(vec[0]) * ratio,
+                  f_u=float(vec[1]) * float(d.speed_u) / (2.0 * np.pi),
+                  phi_min=float(vec[2]), phi_max=float(vec[2]) + window)
+            self.update(d)
 
     def update(self, desc):
         self.desc = desc.copy()
@@ -357,6 +373,28 @@ class Scene:
                                                  C.c_void_p(s.data_ptr())))
         return p, d, L, s
 
+    def jvp_rays(self, n, theta_dot=None, v0_dot=None, seed=1, offset=0):
+        """Per-ray direct trace with tangents (forward mode): returns
+        (exit position, unit direction, optical length, status, and their
+        tangents) along theta_dot (a CUDA float32 tensor of param_count()
+        values, or None) and the per-ray initial-velocity directions v0_dot
+        (CUDA float32 (n, 3), or None)."""
+        import torch
+        dev = f"cuda:{self.ctx.device}"
+        out = [torch.empty(sh, dtype=dt, device=dev) for sh, dt in
+               (((n, 3), torch.float32), ((n, 3), torch.float32), ((n,), torch.float32), ((n,), torch.int32),
+                ((n, 3), torch.float32), ((n, 3), torch.float32), ((n,), torch.float32))]
+        td = None if theta_dot is None else theta_dot.contiguous()
+        if td is not None and (td.dtype != torch.float32 or td.numel() != self.param_count()):
+            raise SimplerError(f"theta_dot must be {self.param_count()} float32 values")
+        vd = None if v0_dot is None else v0_dot.contiguous()
+        if vd is not None and (vd.dtype != torch.float32 or vd.numel() != 3 * n):
+            raise SimplerError("v0_dot must be (n, 3) float32")
+        ptr = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+        self.ctx._bind()
+        self.ctx.check(self.L.simpler_jvp_rays(self.h, n, seed, offset, ptr(td), ptr(vd), *[ptr(t) for t in out]))
+        return tuple(out)
+
     def autotune(self, n, res):
         """Pick block size, steps per chunk and blocks per CU for this scene
         by timing an n-photon render of each candidate; the choice stays on
@@ -365,7 +403,8 @@ class Scene:
         X, Y, Z = res
         self.ctx._bind()
         self.ctx.check(self.L.simpler_autotune(self.h, n, X, Y, Z, C.byref(b), C.byref(s), C.byref(c), C.byref(ms)))
-        return dict(block_size=b.value, steps_per_chunk=s.value, blocks_per_cu=c.value, ms=ms.value)
+        return dict(block_size=b.value, steps_per_chunk=s.value, blocks_per_cu=c.value,
+                    splat=self.ctx.get_tuning()["splat"], ms=ms.value)
 
     def param_count(self):
         """Number of differentiable field parameters (spline coefficients, or

@@ -30,13 +30,15 @@ def gpu_threads():
 
 # GPU test files run in dependency order -- forward parity first, then the
 # layers built on it -- so that under -x a failure names the lowest layer.
-_GPU_ORDER = ["test_gpu_parity.py", "test_gpu_prefilter.py", "test_gpu_host_layer.py", "test_gpu_frontend.py",
-              "test_gpu_batch.py", "test_gpu_adjoint.py"]
+_GPU_ORDER = ["test_gpu_parity.py", "test_gpu_headline.py", "test_gpu_prefilter.py", "test_gpu_host_layer.py", "test_gpu_frontend.py",
+              "test_gpu_batch.py", "test_gpu_adjoint.py", "test_gpu_jvp.py"]
 
 
 def pytest_collection_modifyitems(config, items):
     def rank(item):
         name = os.path.basename(str(item.fspath))
-        return _GPU_ORDER.index(name) if name in _GPU_ORDER else -1
+        if name in _GPU_ORDER:
+            return _GPU_ORDER.index(name)
+        return len(_GPU_ORDER) if name.startswith("test_zz_") else -1  # test_zz_* run last
 
     items.sort(key=rank)

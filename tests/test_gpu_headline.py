@@ -1,0 +1,173 @@
+"""Parity at the full-size scene the throughput is measured on, and on the
+renderer options the smaller tests leave at their defaults.
+
+Full-size scene: 10 mm cube, n0 1.3333, Bessel Delta n 2e-5 at 848 kHz with
+a 20 % duty window, modes 0 and 2, sigma_s 1/mm, g 0.9, collimated 1 mm disk
+source, 5 um ER step, 256 x 256 px x 128 path-length bins over 0-100 mm
+(scenes.bessel_cube_full; reference photon.cpp:266-340, scene.cpp:385-425):
+
+* image: 8 batches of 512 photons on the GPU and on the FP32 oracle, same
+  photons; every bin and the total within 3 sigma, equal event counts;
+* transparent medium: 1e6 rays' exit position, direction and optical
+  length against the FP64 oracle, each within 1e-5 relative;
+* the dead-path cutoff leaves the image unchanged (a photon block whose
+  full-work render is bounded, chosen with the oracle: seed 13, photons
+  32-287, where the cutoff still ends 13 % of the ER steps).
+
+Renderer options (reference scene.cpp:804-805,910-915; photon.cpp:62-67,
+113-114; scene.h:141-164): Fresnel weighting on, bounce decomposition
+(time bin = scattering depth), max_depth and max_pathlength limits, each as
+an image vs the FP32 oracle.  The 64^3 differentiable spline field:
+forward image vs the FP32 oracle, and the field at points up to 1.5 grid
+lengths outside the grid (the mirrored halo and the wrapping path).
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import gpu_threads
+from oracle import oracle as O
+from simpler_amd import engine, scenes
+from simpler_amd._abi import FLAG_NO_DEAD_CUT, FLAG_USE_FRESNEL
+from test_gpu_parity import _batches, _bin_check, log
+
+pytestmark = pytest.mark.gpu
+
+
+def _parity(gpu_ctx, d, tex, res, B, n, seed, what, spline=None, count_tol=0.0):
+    gs = engine.Scene(gpu_ctx, d, tex, spline=spline)
+    osc = O.OracleScene(d, tex, spline=spline)
+    N = B * n
+    g_st, o_st = [], []
+
+    def rg(b):
+        img, st = gs.render(n, res, seed=seed, offset=b * n, n_total=N)
+        g_st.append(st)
+        return img.double().cpu().numpy()
+
+    def ro(b):
+        img, st = osc.render(n, res, seed=seed, offset=b * n, n_total=N, precision=32, threads=gpu_threads(),
+                             cutoff=True)
+        o_st.append(st)
+        return img
+
+    g = _batches(rg, B)
+    o = _batches(ro, B)
+    bad, dtot, tot_sig, nbins = _bin_check(g, o)
+    tot = {k: (sum(s[k] for s in g_st), sum(s[k] for s in o_st))
+           for k in ("photons", "scatters", "direct_splats", "nee_splats", "face_events")}
+    log(f"{what}: bins {nbins} beyond 3 sigma {bad}, total {g.sum():.6g} vs {o.sum():.6g} "
+        f"(diff {dtot:.4g}, sigma {tot_sig:.4g}); counts gpu/oracle {tot}")
+    gs.close()
+    assert nbins > 0
+    assert bad == 0
+    assert dtot <= 3 * tot_sig
+    assert tot["photons"][0] == tot["photons"][1] == N
+    assert tot["direct_splats"][0] == tot["direct_splats"][1]
+    assert abs(tot["scatters"][0] - tot["scatters"][1]) <= count_tol * tot["scatters"][1]
+    ng, no = tot["nee_splats"]
+    assert abs(ng - no) <= 0.002 * no + 2
+    return g, o
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_full_size_image_every_bin_within_3_sigma_of_oracle(gpu_ctx, mode):
+    d, tex, res = scenes.bessel_cube_full(mode)
+    assert res == (256, 256, 128) and abs(d.er_stepsize - 5e-6) < 1e-12
+    _parity(gpu_ctx, d, tex, res, 8, 512, 29 + mode, f"full-size m={mode}", count_tol=1e-3)
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("mode", [0, 2])
+def test_full_size_transparent_1e6_rays_match_fp64(gpu_ctx, mode):
+    d, tex, _ = scenes.transparent_cube(mode=mode)
+    n = 1_000_000
+    sc = engine.Scene(gpu_ctx, d, tex)
+    p, dr, L, s = (t.double().cpu().numpy() for t in sc.trace_rays(n, seed=1))
+    sc.close()
+    p64, d64, L64, s64 = O.OracleScene(d, tex).trace_rays(n, seed=1, precision=64, threads=gpu_threads())
+    assert np.array_equal(s, s64) and (s == 1).all()
+    ep = np.abs(p - p64).max() / 0.005
+    ed = np.abs(dr - d64).max()
+    eL = (np.abs(L - L64) / L64).max()
+    log(f"1e6 rays m={mode}: position {ep:.2e}, direction {ed:.2e}, optical length {eL:.2e} (max over rays)")
+    assert ep < 1e-5 and ed < 1e-5 and eL < 1e-5
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_full_size_dead_path_cutoff_leaves_gpu_image_unchanged(gpu_ctx, mode):
+    d, tex, res = scenes.bessel_cube_full(mode)
+    seed, off, n = 13, 32, 256
+    osc = O.OracleScene(d, tex)
+    _, ofull = osc.render(n, res, seed=seed, offset=off, n_total=n, precision=32, threads=gpu_threads(), cutoff=False)
+    _, ocut = osc.render(n, res, seed=seed, offset=off, n_total=n, precision=32, threads=gpu_threads(), cutoff=True)
+    assert ofull["er_steps"] < 2e7  # the block's full-work render is bounded
+    sc = engine.Scene(gpu_ctx, d, tex)
+    cut, scut = sc.render(n, res, seed=seed, offset=off, n_total=n)
+    d2 = d.copy()
+    d2.flags |= FLAG_NO_DEAD_CUT
+    sc2 = engine.Scene(gpu_ctx, d2, tex)
+    full, sfull = sc2.render(n, res, seed=seed, offset=off, n_total=n)
+    cut, full = cut.double().cpu().numpy(), full.double().cpu().numpy()
+    log(f"cutoff m={mode}: steps gpu {scut['er_steps']} / {sfull['er_steps']} (cut / full), oracle "
+        f"{ocut['er_steps']} / {ofull['er_steps']}; image max diff {np.abs(cut - full).max():.3g} of {full.max():.3g}")
+    sc.close()
+    sc2.close()
+    assert full.sum() > 0
+    assert np.abs(cut - full).max() <= 1e-6 * full.max()
+    assert scut["er_steps"] < sfull["er_steps"]  # the cutoff acted
+    assert abs(sfull["er_steps"] - ofull["er_steps"]) < 1e-3 * ofull["er_steps"]
+    assert abs(scut["er_steps"] - ocut["er_steps"]) < 1e-3 * ocut["er_steps"]
+
+
+@pytest.mark.parametrize("option", ["fresnel", "bounce", "max_depth", "max_pathlength"])
+def test_renderer_options_within_3_sigma_of_oracle(gpu_ctx, option):
+    d, tex, res = scenes.bessel_cube_small()
+    if option == "fresnel":
+        d.flags |= FLAG_USE_FRESNEL
+    elif option == "bounce":
+        d.set(use_bounce_decomposition=1)
+    elif option == "max_depth":
+        d.set(max_depth=3)
+    else:
+        d.set(max_pathlength=0.003)
+    g, o = _parity(gpu_ctx, d, tex, res, 8, 48, 41, option)
+    if option == "bounce":  # every splat lands in the bin of its depth
+        depth_mass = g.sum((0, 2, 3))
+        assert depth_mass[0] > 0 and depth_mass[1] > 0
+    if option == "fresnel":  # the weight carries ior^2 (1 - F) at entry and the exit transmission
+        plain, _ = engine.Scene(gpu_ctx, scenes.bessel_cube_small()[0], tex).render(8 * 48, res, seed=41)
+        ratio = g.sum() / float(plain.double().sum())
+        log(f"fresnel on / off total {ratio:.4f}")
+        assert ratio < 0.99
+
+
+def test_spline_64cube_image_within_3_sigma_of_oracle(gpu_ctx):
+    d, tex, res, spl = scenes.spline_cube_ready(n=64)
+    _parity(gpu_ctx, d, tex, res, 8, 48, 5, "spline 64^3", spline=spl, count_tol=1e-3)
+
+
+def test_spline_field_outside_the_grid_matches_oracle(gpu_ctx):
+    """Points up to 1.5 grid lengths beyond the grid box: within one cell the
+    padded copy's mirrored halo, further out the per-tap mirror wrapping."""
+    d, tex, _, (vals, dims, lo, hi) = scenes.spline_cube(n=24)
+    coeff = O.spline_build3d(vals.reshape(-1), dims).astype(np.float32)
+    spl = (coeff, dims, lo, hi)
+    d.set(egap_end_loc_x=1.0, sgap_begin_loc_x=-1.0)  # no gap rule: evaluate everywhere
+    rng = np.random.default_rng(12)
+    span = hi[0] - lo[0]
+    pts = rng.uniform(lo[0] - 1.5 * span, hi[0] + 1.5 * span, (4000, 3))
+    cell = span / (dims[0] - 1)
+    near = rng.uniform(lo[0] - 1.2 * cell, hi[0] + 1.2 * cell, (4000, 3))  # the halo band
+    pts = np.concatenate([pts, near])
+    n_o, g_o = O.OracleScene(d, tex, spline=spl).field(pts)
+    sc = engine.Scene(gpu_ctx, d, tex, spline=spl)
+    n_g, g_g = sc.field(torch.tensor(pts, dtype=torch.float32, device="cuda"))
+    n_g, g_g = n_g.double().cpu().numpy(), g_g.double().cpu().numpy()
+    sc.close()
+    outside = (np.abs(pts) > hi[0]).any(1)
+    log(f"spline outside the grid: {outside.sum()} points outside, dn {np.abs(n_g - n_o).max():.2e}, "
+        f"dg {np.abs(g_g - g_o).max() / np.abs(g_o).max():.2e}")
+    assert outside.sum() > 4000
+    assert np.abs(n_g - n_o).max() < 4e-7
+    assert np.abs(g_g - g_o).max() < 1e-4 * np.abs(g_o).max()

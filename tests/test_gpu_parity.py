@@ -248,6 +248,56 @@ def test_chunked_render_adds_up_on_gpu(gpu_ctx):
     assert np.abs(again.double().cpu().numpy() - full).max() < 1e-4 * full.max()
 
 
+def test_stream_switch_is_ordered(gpu_ctx):
+    """A context moved to another stream while its previous render is still
+    running: the second render must wait for the first (they share the work
+    queue and the counters), so two half renders on two streams add up to the
+    one-shot render."""
+    d, tex, res = scenes.bessel_cube(step=1e-5, n_max=1e-5, mode=0, res=(32, 32, 16))
+    gs = engine.Scene(gpu_ctx, d, tex)
+    N = 1 << 16
+    ref, st_ref = gs.render(N, res, seed=19, n_total=N)
+    ref = ref.double().cpu().numpy()
+    img = torch.zeros((res[2], res[1], res[0]), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    with torch.cuda.stream(sa):
+        gs.render(N // 2, res, seed=19, offset=0, n_total=N, image=img, accumulate=True, stats=False)
+    with torch.cuda.stream(sb):
+        gs.render(N // 2, res, seed=19, offset=N // 2, n_total=N, image=img, accumulate=True, stats=False)
+        st = gpu_ctx.read_stats()
+    torch.cuda.synchronize()
+    two = img.double().cpu().numpy()
+    log(f"stream switch: photons of the second launch {st['photons']}, totals {two.sum():.6g} / {ref.sum():.6g}")
+    assert st["photons"] == N // 2
+    assert abs(two.sum() - ref.sum()) < 1e-5 * ref.sum()
+    assert np.abs(two - ref).max() < 1e-4 * ref.max()
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_wave_aggregated_splat_equals_per_lane_splat(gpu_ctx, mode):
+    """The two splat strategies (per-lane atomics, wave-aggregated atomics)
+    render the same photons into the same image, up to the order of the
+    float additions."""
+    d, tex, res = scenes.bessel_cube_full(mode)
+    sc = engine.Scene(gpu_ctx, d, tex)
+    tun = gpu_ctx.get_tuning()
+    n = 1 << 17
+    imgs, sts = [], []
+    for strategy in (0, 1):
+        gpu_ctx.set_tuning(splat=strategy)
+        img, st = sc.render(n, res, seed=21)
+        imgs.append(img.double().cpu().numpy())
+        sts.append(st)
+    gpu_ctx.set_tuning(splat=tun["splat"])
+    a, b = imgs
+    log(f"splat strategies m={mode}: totals {a.sum():.7g} / {b.sum():.7g}, max diff {np.abs(a - b).max():.3g}")
+    sc.close()
+    assert sts[0] == sts[1]
+    assert a.sum() > 0 and abs(a.sum() - b.sum()) <= 1e-5 * a.sum()
+    assert np.abs(a - b).max() <= 1e-5 * a.max()
+
+
 def test_dead_path_cutoff_leaves_gpu_image_unchanged(gpu_ctx):
     d, tex, res = scenes.bessel_cube_small()
     n, seed = 48, 13
