@@ -249,6 +249,23 @@ int simpler_render_host(simpler_scene *scene, int64_t n_photons,
                         float *image_host, int x_res, int y_res, int z_res,
                         int accumulate, simpler_stats *stats_host);
 
+/* simpler_render_host without any device allocation: the same render, with
+ * the launch's work queue, counters and device image staging in a fixed
+ * file-scope device array of the library (no counterpart in the reference;
+ * for callers that cannot allocate device memory: tight serving loops,
+ * harnesses that forbid hipMalloc).  The array holds a 256 B header plus at
+ * most 8,388,608 image floats (32 MiB, a 256 x 256 x 128 image); larger
+ * images fail with SIMPLER_ERR_INVALID.  Each device the library runs on
+ * holds one copy until the process ends, and every call on that device
+ * shares it (the last call's contents stay): calls that overlap on one
+ * device -- on two streams or two contexts at once -- race and must be
+ * serialised by the caller.  Synchronous. */
+int simpler_render_host_noalloc(simpler_scene *scene, int64_t n_photons,
+                                int64_t n_total, uint64_t seed,
+                                uint64_t photon_offset, float *image_host,
+                                int x_res, int y_res, int z_res, int accumulate,
+                                simpler_stats *stats_host);
+
 /*
  * Per-ray record of the direct (unscattered) trace, the transparent-medium
  * parity mode: for each photon i in [photon_offset, photon_offset+n_rays) the

@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1138,6 +1139,49 @@ __global__ void k_bessel_field_eval(DevScene S, int n, const float *pts, float s
 // ======================================================================
 using namespace simpler;
 
+// Scratch of the no-allocation render (simpler_render_host_noalloc): a
+// fixed device array that the code object brings with it, so the entry
+// point needs no hipMalloc.  Layout: 32 u64 header (work queue at [0],
+// counters at [8..15]), then the device image staging of at most
+// kNoAllocPixels floats (32 MiB: a 256 x 256 x 128 image).  Each device the
+// library runs on holds its own copy until the process ends; all calls on
+// one device share it.
+constexpr size_t kNoAllocHeaderWords = 32;
+constexpr size_t kNoAllocPixels = (size_t)256 * 256 * 128;
+static __device__ __align__(256) unsigned long long scratch[kNoAllocHeaderWords + kNoAllocPixels / 2];
+
+// The array's device address on each device.  The host-side name `scratch`
+// is not a device address: host code uses these pointers, obtained once per
+// device (with that device current) by hipGetSymbolAddress.
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned long long *g_scratch_dev[kMaxDevices];
+static hipError_t g_scratch_err[kMaxDevices];
+static bool g_scratch_done[kMaxDevices];
+
+// Called by simpler_ctx_create with the device current.
+static void scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_done[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    g_scratch_dev[device] = e == hipSuccess ? static_cast<unsigned long long *>(p) : nullptr;
+    if (e != hipSuccess) (void)hipGetLastError();
+    g_scratch_done[device] = true;
+}
+
+static unsigned long long *scratch_ptr(int device, hipError_t *err) {
+    if (device < 0 || device >= kMaxDevices) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    *err = g_scratch_done[device] ? g_scratch_err[device] : hipErrorNotInitialized;
+    return g_scratch_dev[device];
+}
+
 struct simpler_ctx {
     int device = 0;
     hipStream_t own = nullptr;
@@ -1549,6 +1593,7 @@ int simpler_ctx_create(int device, simpler_ctx **out) {
         return SIMPLER_ERR_HIP;
     }
     (void)hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, device);
+    scratch_init(device);
     if (hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->handoff, hipEventDisableTiming) != hipSuccess ||
         hipMalloc(&ctx->d_scratch, 16 * sizeof(unsigned long long)) != hipSuccess) {
@@ -2130,6 +2175,33 @@ int simpler_render_host(simpler_scene *sc, int64_t n_photons, int64_t n_total, u
     (void)hipStreamSynchronize(ctx->cur);
     (void)hipFree(d);
     return rc;
+}
+
+int simpler_render_host_noalloc(simpler_scene *sc, int64_t n_photons, int64_t n_total, uint64_t seed,
+                                uint64_t offset, float *image_host, int X, int Y, int Z, int accumulate,
+                                simpler_stats *stats) {
+    if (!sc || !image_host) return SIMPLER_ERR_INVALID;
+    simpler_ctx *ctx = sc->ctx;
+    if (X <= 0 || Y <= 0 || Z <= 0) return fail(ctx, SIMPLER_ERR_INVALID, "image resolution must be positive");
+    if ((double)X * Y * Z > (double)kNoAllocPixels)
+        return fail(ctx, SIMPLER_ERR_INVALID,
+                    "no-alloc render: the image exceeds the 8,388,608-pixel (32 MiB) staging buffer");
+    hipError_t se = hipSuccess;
+    unsigned long long *words = scratch_ptr(ctx->device, &se);
+    if (!words)
+        return fail(ctx, SIMPLER_ERR_STATE, std::string("no-alloc render: no scratch address for the device: ") +
+                                                hipGetErrorString(se));
+    float *img = reinterpret_cast<float *>(words + kNoAllocHeaderWords);
+    const size_t bytes = (size_t)X * Y * Z * sizeof(float);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (accumulate) HIP_TRY(ctx, hipMemcpyAsync(img, image_host, bytes, hipMemcpyHostToDevice, ctx->cur));
+    const int rc = render_impl(sc, n_photons, n_total, seed, offset, img, X, Y, Z, accumulate, false, nullptr,
+                               nullptr, nullptr, nullptr, words);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(image_host, img, bytes, hipMemcpyDeviceToHost, ctx->cur));
+    if (stats) return read_counters(ctx, words, stats);
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
+    return SIMPLER_OK;
 }
 
 int simpler_trace_rays(simpler_scene *sc, int64_t n_rays, uint64_t seed, uint64_t offset, float *out_pos,

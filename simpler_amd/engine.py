@@ -79,6 +79,8 @@ def load_library():
                                  C.c_int, C.POINTER(Stats)]
     L.simpler_render_host.argtypes = [P, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, fptr, C.c_int, C.c_int,
                                       C.c_int, C.c_int, C.POINTER(Stats)]
+    L.simpler_render_host_noalloc.argtypes = [P, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, fptr, C.c_int,
+                                              C.c_int, C.c_int, C.c_int, C.POINTER(Stats)]
     L.simpler_trace_rays.argtypes = [P, C.c_int64, C.c_uint64, C.c_uint64, P, P, P, P]
     L.simpler_jvp_rays.argtypes = [P, C.c_int64, C.c_uint64, C.c_uint64, P, P, P, P, P, P, P, P, P]
     L.simpler_field_eval.argtypes = [P, C.c_int, P, C.c_float, C.c_float, P, P]
@@ -102,7 +104,7 @@ def load_library():
                  "simpler_ctx_set_tuning", "simpler_ctx_get_tuning", "simpler_ctx_set_splat", "simpler_ctx_get_splat", "simpler_scene_create", "simpler_scene_destroy",
                  "simpler_scene_update", "simpler_scene_get_desc", "simpler_scene_set_sources",
                  "simpler_scene_set_spline", "simpler_render", "simpler_render_host", "simpler_trace_rays", "simpler_jvp_rays",
-                 "simpler_field_eval", "simpler_read_stats", "simpler_batch_create", "simpler_batch_render",
+                 "simpler_field_eval", "simpler_read_stats", "simpler_render_host_noalloc", "simpler_batch_create", "simpler_batch_render",
                  "simpler_batch_graph_capture", "simpler_batch_graph_launch", "simpler_batch_destroy",
                  "simpler_adjoint_param_count", "simpler_adjoint_render", "simpler_adjoint_rays",
                  "simpler_autotune", "simpler_batch_create_sources", "simpler_batch_adjoint",
@@ -356,6 +358,21 @@ class Scene:
         self.ctx._bind()
         self.ctx.check(self.L.simpler_render_host(self.h, n, n_total or n, seed, offset, _fp(img), X, Y, Z, 0,
                                                   C.byref(st)))
+        return img.reshape(Z, Y, X), st.as_dict()
+
+    def render_host_noalloc(self, n, res, seed=1, offset=0, n_total=None, image=None, accumulate=False):
+        """render_host through the entry point that allocates no device
+        memory (its staging lives in the library's fixed device array; at
+        most 8,388,608 pixels).  image: optional float32 numpy array (Z, Y, X)
+        to fill, or to add to with accumulate."""
+        X, Y, Z = res
+        img = np.zeros(X * Y * Z, np.float32) if image is None else image
+        if img.dtype != np.float32 or not img.flags["C_CONTIGUOUS"] or img.size != X * Y * Z:
+            raise SimplerError("image must be a contiguous float32 array of X*Y*Z elements")
+        st = Stats()
+        self.ctx._bind()
+        self.ctx.check(self.L.simpler_render_host_noalloc(self.h, n, n_total or n, seed, offset, _fp(img), X, Y, Z,
+                                                          int(accumulate), C.byref(st)))
         return img.reshape(Z, Y, X), st.as_dict()
 
     def trace_rays(self, n, seed=1, offset=0):
