@@ -49,6 +49,7 @@ struct AdjState {
     f3 pb;         //   pa dn/dtheta + pb . dgrad/dtheta
     float mu, gL;  // path-length adjoint (see header), optical-length adjoint
     f3 ldin;       // NEE: adjoint of the main-path incoming velocity
+    f3 src;        // the photon's source point: the exact start of the main path
     float rh;      // reverse iterator: step size, level, steps left, counts
     int rlev, rleft, rfinal, racc;
     uint32_t rcnt0, rpat;
@@ -162,8 +163,10 @@ __device__ __forceinline__ f3 reflect_vec(f3 x, f3 nrm) { return x - (2.f * dot(
 // The start point is p = q - h vh / n(p), solved with n evaluated at
 // q - h vh / n(q) (the error is O(h^3 |grad n|^2), far below FP32).
 // Returns false when the segment start is reached.
-template <int FIELD>
-__device__ __forceinline__ bool rev_step(Hot &h, AdjState &a, const DevScene &S, const FieldP &F) {
+// CELL: the spline taps come from the lane's register copy of the current cell.
+template <int FIELD, bool CELL = false>
+__device__ __forceinline__ bool rev_step(Hot &h, AdjState &a, const DevScene &S, const FieldP &F,
+                                         SplineCell *cell = nullptr) {
     const float hs = a.rh;
     const float hh = 0.5f * hs;
     a.lp = a.lp + hh * symmul(a.H, a.lv);
@@ -178,7 +181,10 @@ __device__ __forceinline__ bool rev_step(Hot &h, AdjState &a, const DevScene &S,
     float n1;
     f3 g1;
     Sym3 H1;
-    field_eval_h<FIELD>(S, F, pt, h.scaling, h.tau, n1, g1, H1);
+    if constexpr (FIELD == kSpline && CELL)
+        spline_eval_h<true>(S, pt, n1, g1, H1, cell);
+    else
+        field_eval_h<FIELD>(S, F, pt, h.scaling, h.tau, n1, g1, H1);
     const float inv = 1.0f / n1;
     const f3 p0 = h.p - (hs * inv) * vh;
     const float s = hs * inv * inv * dot(vh, a.lp);

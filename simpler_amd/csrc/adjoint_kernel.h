@@ -60,7 +60,7 @@ __device__ __forceinline__ void rev_end(Lane &L) {
     uint32_t cnt0, pat, kind;
     if (a.rphase == RP_MAIN) {
         if (a.rseg == 0) {  // the source: v0 = n(p0) d0 with p0, d0 fixed
-            h.p = L.c.src;
+            h.p = a.src;
             field_vjp<FIELD>(S, F, h.p, h.scaling, h.tau, a.pa, a.pb, a.G);
             const float vl = len(h.v);
             field_vjp<FIELD>(S, F, h.p, h.scaling, h.tau, dot(a.lv, h.v) / vl, mk3(0.f, 0.f, 0.f), a.G);
@@ -158,6 +158,8 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
     h.state = ST_NEED;
     h.dead = false;
     AdjState a;
+    SplineCell scell;  // spline kernels: register copy of the current cell's taps
+    scell.key = -1;
     a.F = 0.0;
     a.rsteps = a.nsubrev = a.nmainrev = a.dropped = 0;
     a.pending = a.trunc_main = a.trunc_sub = false;
@@ -221,9 +223,9 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
         uint32_t sl = h.steps_left;
         for (int k = 0; k < A.spc; ++k) {
             if (m)
-                m = !march_step<FIELD, true>(h, sl, S, F, Lcut);
+                m = !march_step<FIELD, true, -1, true, -1, FIELD == kSpline>(h, sl, S, F, Lcut, &scell);
             else if (r)
-                r = rev_step<FIELD>(h, a, S, F);
+                r = rev_step<FIELD, FIELD == kSpline>(h, a, S, F, &scell);
         }
         h.steps_left = sl;
         if (h.state == ST_MARCH && !m) h.state = ST_SEG_END;

