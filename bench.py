@@ -31,10 +31,11 @@ F_STEP = {0: 92, 1: 126, 2: 148}
 F_EVENT = dict(scatters=80, nee_splats=127, direct_splats=100, face_events=45, photons=40)
 FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per photon of k_render (mode 0), from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE of one 2^20-photon launch (8,410.6 and 1,316,608.4 KiB),
-# scaled by the counters' reading for a 1 GiB device copy (x2.00 fetch,
-# x1.00 write): almost all of it is the splat atomics.
-TRAFFIC_BYTES_PER_PHOTON = 1302.18
+# and WRITE_SIZE of one 2^24-photon launch at the bench's launch shape
+# (5,846.9 and 20,943,020.8 KiB), scaled by the counters' reading for a
+# 1 GiB device copy (x2.00 fetch, x1.00 write): almost all of it is the
+# splat atomics.
+TRAFFIC_BYTES_PER_PHOTON = 1278.97
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per sample of the three prefilter passes on a 512^3 grid
 # (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of one call, same scaling):
@@ -282,6 +283,8 @@ def adjoint_bench(args):
     per_fd = 2.0 * t_fwd
     out = dict(workload=("64^3 spline field, sigma_s 2/mm, g 0.9, 5 um step, 64x64x32 sensor, "
                          "throughput into a 1 mm aperture"),
+               schedule=("deferred: re-trace queueing the backward runs, then full-wave sub-path and main-path "
+                         "kernels" if ctx.get_adjoint_schedule()["schedule"] == 1 else "inline"),
                paths=N, parameters=n_par, forward_s=t_fwd, adjoint_s=t_adj, adjoint_over_forward=t_adj / t_fwd,
                objective=F, objective_render=F_img, forward_paths_per_s=N / t_fwd, adjoint_paths_per_s=N / t_adj,
                fd_check=dict(scene="same field and objective, index-matched faces, no time gate",

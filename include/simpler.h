@@ -302,6 +302,30 @@ int simpler_jvp_rays(simpler_scene *scene, int64_t n_rays, uint64_t seed,
                      int32_t *out_status_dev, float *tan_pos_dev,
                      float *tan_dir_dev, float *tan_len_dev);
 
+/* Deterministic next-event connection: replaces Scene::computefdfNEE
+ * (scene.cpp:512-574, scene.h:291).  For each ray i (device arrays, 3
+ * floats per ray): start at p1 with the velocity v1 scaled to |v| = n(p1),
+ * march with the forward-mode leapfrog of er_derivativestep (scene.cpp:
+ * 351-359) until x < p2.x, then bisect the last step ceil(precision /
+ * log10 2) times (a half step is kept while it stays at x > p2.x).  The
+ * reference's caller-initialised derivatives start as dp/dv1 = 0,
+ * dv/dv1 = I.  Outputs: err_dev = p - p2 (3 per ray) and jac_dev = the
+ * transpose of dQ/dv1 (9 per ray, row-major: row j is the derivative of the
+ * crossing point Q with respect to v1_j), where dQ/dv1 = (I - t e_x^T / t_x)
+ * dp/dv1, t = v / n(p), adds the crossing time's dependence on v1.  The
+ * field is the scene's at time scaling `scaling` (Bessel) and duty phase
+ * `tau` (point sources with time coupling).  The reference's HessianRIF
+ * has no definition at HEAD (its predecessors: the unscaled Bessel
+ * Hessian of scene.cpp:260-320 and a zero matrix for point sources,
+ * scene.cpp:197-201); this uses each field's full Hessian.  The Bessel
+ * component form (general axis off) is accepted where it equals the general
+ * form (axis -x, or +x with m = 0).  The reference has no caller; the loop
+ * runs at most 1e5 steps, as there.  Asynchronous on the ctx stream. */
+int simpler_nee_connect(simpler_scene *scene, int64_t n_rays,
+                        const float *p1_dev, const float *v1_dev,
+                        const float *p2_dev, float scaling, float tau,
+                        float *err_dev, float *jac_dev);
+
 /* Refractive index n and its gradient at n_pts points (device arrays:
  * pts 3*n_pts, out_n n_pts, out_grad 3*n_pts) for the per-photon amplitude
  * scaling s and duty phase tau -- the field evaluation of the march
@@ -441,6 +465,25 @@ int simpler_ctx_get_tuning(simpler_ctx *ctx, int *block_size,
  * (uz = +z, ux = +x); other renders splat per lane whatever the setting. */
 int simpler_ctx_set_splat(simpler_ctx *ctx, int strategy);
 int simpler_ctx_get_splat(simpler_ctx *ctx, int *strategy);
+
+/* Schedule of simpler_adjoint_render's backward runs (no reference
+ * counterpart: the reference has no adjoint).  schedule 0 = inline: a lane
+ * that splats into the objective runs the sub-path backwards at once, and the
+ * photon's main path when it ends, while the other lanes of its wave march
+ * on; 1 (default) = deferred: the re-trace queues each such sub-path (its log
+ * and adjoint seed) and each photon's main-path log, then a second kernel runs
+ * the queued sub-paths and a third the main paths, every lane reversing.
+ * queue_capacity c (0 keeps the current value, default 2^23; else
+ * 64 <= c <= 2^26) sizes the queues of one chunk of photons: c sub-paths
+ * with c sub-path records, c main paths with 8 c main-path records, 768 c
+ * bytes of device memory (6 GiB at the default), allocated on first use and
+ * kept by the context.  The chunk starts at 2^22 photons, halves when a
+ * chunk's queues overflow (the chunk is then traced again) and doubles, up
+ * to 2^24, after a chunk that filled no queue past 40 %; at 2^14 photons an
+ * overflowing chunk runs inline.  Both schedules give the same objective
+ * and counts, and the same gradient up to the order of float additions. */
+int simpler_ctx_set_adjoint_schedule(simpler_ctx *ctx, int schedule, int64_t queue_capacity);
+int simpler_ctx_get_adjoint_schedule(simpler_ctx *ctx, int *schedule, int64_t *queue_capacity);
 
 /* Autotuner (no reference counterpart: the reference hard-codes its launch,
  * cuda_renderer.cu:763-767): times an n_photons render of the scene at the

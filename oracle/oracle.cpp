@@ -474,6 +474,167 @@ static V3<R> dRIF(const Scene<R> &S, const V3<R> &p, const TimeState<R> &ts) {
     }
 }
 
+/* Hessian of n (symmetric, H[i][j] = d2n / dx_i dx_j), for the
+ * deterministic next-event connection.  The reference's HessianRIF has no
+ * definition at HEAD (its predecessors in scene.cpp: an unscaled Bessel
+ * Hessian, 260-320, and a zero matrix for point sources, 197-201); here each
+ * field's full second derivative, derived from its closed form:
+ *   Bessel  f = A J_m(k r) cos(m phi), A = n_max s; in the frame (r^, t^ =
+ *           uz x r^): H = f_rr r^r^T + (f_r / r + f_pp / r^2) t^t^T
+ *           + (f_rp / r - f_p / r^2)(r^t^T + t^r^T), with J'' from Bessel's
+ *           equation J'' = -J'/x - (1 - m^2 / x^2) J (glibc jn);
+ *   sources f = cos(psi) / r per source and harmonic: H = f'' r^r^T
+ *           + (f' / r)(I - r^r^T);
+ *   spline  the B-spline kernels' second derivatives (spline.h:33-53). */
+template <class R>
+struct M3 {
+    R m[3][3];
+};
+
+template <class R>
+static M3<R> bessel_HRIF(const Scene<R> &S, const V3<R> &q, R scaling) {
+    const V3<R> uz = v3<R>(S.d.axis_uz), ux = v3<R>(S.d.axis_ux), pu = v3<R>(S.d.p_u);
+    const V3<R> p_axis = pu + dot(q - pu, uz) * uz;
+    V3<R> p = q - p_axis;
+    R r = p.length();
+    const R phi = std::atan2(dot(cross(ux, p), uz), dot(p, ux));
+    if (r < K<R>::EPS) {
+        p = K<R>::EPS * (ux + cross(uz, ux));
+        r = K<R>::EPS;
+    }
+    const int m = S.d.mode;
+    const R x = S.k_r * r;
+    const R J = (R)besselj(m, (double)x);
+    const R dJ = (R)m / x * J - (R)besselj(m + 1, (double)x);
+    const R d2J = -dJ / x - (R(1) - (R)(m * m) / (x * x)) * J;
+    const R A = (R)S.d.n_max * scaling, C = std::cos((R)m * phi), Sn = std::sin((R)m * phi), k = S.k_r;
+    const R f_r = A * k * dJ * C, f_p = -A * (R)m * J * Sn;
+    const R f_rr = A * k * k * d2J * C, f_pp = -A * (R)(m * m) * J * C, f_rp = -A * (R)m * k * dJ * Sn;
+    const V3<R> rh = p / r, th = cross(uz, rh);
+    const R a = f_rr, b = f_r / r + f_pp / (r * r), c = f_rp / r - f_p / (r * r);
+    M3<R> H;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) H.m[i][j] = a * rh[i] * rh[j] + b * th[i] * th[j] + c * (rh[i] * th[j] + th[i] * rh[j]);
+    return H;
+}
+
+template <class R>
+static M3<R> sources_HRIF(const Scene<R> &S, const V3<R> &p, R tau) {
+    M3<R> H{};
+    for (int c = 1; c <= S.n_coeff; c++) {
+        for (int n = 0; n < S.n_src; n++) {
+            const V3<R> ci((R)S.src_pos[3 * n], (R)S.src_pos[3 * n + 1], (R)S.src_pos[3 * n + 2]);
+            const V3<R> dv = p - ci;
+            const R r = dv.length();
+            const R kc = S.k_r * (R)c;
+            const R psi = kc * r + (R)S.src_phase[n] - (R)S.d.time_coupling * tau;
+            const R g = std::cos(psi) + kc * r * std::sin(psi);
+            const R f1 = -g / (r * r);
+            const R f2 = R(2) * g / (r * r * r) - kc * kc * std::cos(psi) / r;
+            const R w = (R)S.src_amp[n] * (R)S.coeff_w[c - 1];
+            const V3<R> rh = dv / r;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    H.m[i][j] += w * (f2 * rh[i] * rh[j] + (f1 / r) * ((i == j ? R(1) : R(0)) - rh[i] * rh[j]));
+        }
+    }
+    return H;
+}
+
+template <class R>
+static M3<R> HRIF(const Scene<R> &S, const V3<R> &p, const TimeState<R> &ts) {
+    M3<R> H{};
+    if (outside_gap(S, p)) return H;
+    switch (S.d.field_kind) {
+        case SIMPLER_FIELD_BESSEL: return bessel_HRIF(S, p, ts.scaling);
+        case SIMPLER_FIELD_SOURCES: return sources_HRIF(S, p, ts.tau);
+        default:
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    int d[3] = {0, 0, 0};
+                    d[i]++;
+                    d[j]++;
+                    H.m[i][j] = S.spl.value(p, d[0], d[1], d[2]);
+                }
+            return H;
+    }
+}
+
+/* scene.cpp:512-574 computefdfNEE with er_derivativestep (351-359) and
+ * dV / d2V / d2Path (scene.h:253-272); dpdv0 and dvdv0 enter as 0 and I.
+ * Matrices: P[i][j] = dp_i / dv1_j, V[i][j] = dv_i / dv1_j. */
+template <class R>
+static void nee_connect(const Scene<R> &S, const TimeState<R> &ts, const V3<R> &v_i, const V3<R> &p1,
+                        const V3<R> &p2, int nbis, V3<R> &error, M3<R> &derror) {
+    R h = (R)S.d.er_stepsize;
+    V3<R> p = p1, v = v_i;
+    M3<R> P{}, V{};
+    const R r = RIF(S, p, ts);
+    const R n1 = v.length(), n2 = n1 * n1, n3 = n2 * n1;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) V.m[i][j] = ((i == j ? n2 : R(0)) - v[i] * v[j]) / n3 * r;
+    v = v / n1 * r;
+    auto step = [&](V3<R> &pp, V3<R> &vv, M3<R> &PP, M3<R> &VV, R hs) {
+        const R hh = R(0.5) * hs;
+        vv += hh * dRIF(S, pp, ts);
+        M3<R> H = HRIF(S, pp, ts);
+        M3<R> T{};
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                for (int k = 0; k < 3; k++) T.m[i][j] += H.m[i][k] * PP.m[k][j];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) VV.m[i][j] += hh * T.m[i][j];
+        pp += hs * vv / RIF(S, pp, ts);
+        const R n = RIF(S, pp, ts);
+        const V3<R> g = dRIF(S, pp, ts);
+        R gP[3] = {0, 0, 0};
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) gP[j] += g[k] * PP.m[k][j];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) PP.m[i][j] += hs * (-(vv[i] * gP[j]) / (n * n) + VV.m[i][j] / n);
+        vv += hh * dRIF(S, pp, ts);
+        H = HRIF(S, pp, ts);
+        M3<R> T2{};
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                for (int k = 0; k < 3; k++) T2.m[i][j] += H.m[i][k] * PP.m[k][j];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) VV.m[i][j] += hh * T2.m[i][j];
+    };
+    V3<R> oldp = p, oldv = v;
+    M3<R> oldP = P, oldV = V;
+    for (int it = 0; it < 100000; it++) {
+        oldp = p;
+        oldv = v;
+        oldP = P;
+        oldV = V;
+        step(p, v, P, V, h);
+        if (p.x < p2.x) {
+            while (nbis > 0) {
+                nbis--;
+                p = oldp;
+                v = oldv;
+                P = oldP;
+                V = oldV;
+                h = h / 2;
+                step(p, v, P, V, h);
+                if (p.x > p2.x) {
+                    oldp = p;
+                    oldv = v;
+                    oldP = P;
+                    oldV = V;
+                }
+            }
+            break;
+        }
+    }
+    const V3<R> t = v / RIF(S, p, ts);
+    /* dQdV = (I - t dgdQ^T / (dgdQ . t)) P, dgdQ = (-1, 0, 0); derror = dQdV^T */
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) derror.m[j][i] = P.m[i][j] - t[i] * P.m[0][j] / t.x;
+    error = p - p2;
+}
+
 /* ------------------------------------------------------------- counters */
 struct Counters {
     uint64_t photons = 0, steps = 0, scatters = 0, nee = 0, direct = 0, faces = 0, segments = 0, aborted = 0;
@@ -1315,6 +1476,43 @@ int oracle_field_eval(void *hv, int precision, int npts, const double *pts, doub
             out_g[3 * i + 1] = g.y;
             out_g[3 * i + 2] = g.z;
         }
+    }
+    return 0;
+}
+
+/* Hessian of n at points (H row-major, 9 per point). */
+int oracle_field_hessian(void *hv, int npts, const double *pts, double scaling, double tau, double *out_h) {
+    const OracleSceneHandle &h = *(OracleSceneHandle *)hv;
+    Scene<double> S;
+    fill_scene(S, h, 1);
+    TimeState<double> ts{scaling, tau};
+    for (int i = 0; i < npts; i++) {
+        const M3<double> H = HRIF(S, V3<double>(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]), ts);
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) out_h[9 * i + 3 * a + b] = H.m[a][b];
+    }
+    return 0;
+}
+
+/* computefdfNEE for n rays (FP64): err 3 and derror 9 (row-major) per ray. */
+int oracle_nee_connect(void *hv, int n, const double *p1, const double *v1, const double *p2, double scaling,
+                       double tau, double *err, double *jac) {
+    const OracleSceneHandle &h = *(OracleSceneHandle *)hv;
+    Scene<double> S;
+    fill_scene(S, h, 1);
+    TimeState<double> ts{scaling, tau};
+    const int nbis = (int)std::ceil((double)S.d.precision / std::log10(2.0)); /* scene.cpp:520 */
+    for (int i = 0; i < n; i++) {
+        V3<double> e;
+        M3<double> J;
+        nee_connect(S, ts, V3<double>(v1[3 * i], v1[3 * i + 1], v1[3 * i + 2]),
+                    V3<double>(p1[3 * i], p1[3 * i + 1], p1[3 * i + 2]),
+                    V3<double>(p2[3 * i], p2[3 * i + 1], p2[3 * i + 2]), nbis, e, J);
+        err[3 * i] = e.x;
+        err[3 * i + 1] = e.y;
+        err[3 * i + 2] = e.z;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) jac[9 * i + 3 * a + b] = J.m[a][b];
     }
     return 0;
 }

@@ -53,6 +53,8 @@ def lib(fast=False):
     L.oracle_trace_rays.argtypes = [P, C.c_int, C.c_int64, C.c_uint64, C.c_uint64, dp, dp, dp, ip, C.c_int]
     L.oracle_field_eval.argtypes = [P, C.c_int, C.c_int, dp, C.c_double, C.c_double, dp, dp]
     L.oracle_er_steps.argtypes = [P, C.c_int, dp, dp, C.c_double, C.c_long, C.c_double, C.c_double]
+    L.oracle_field_hessian.argtypes = [P, C.c_int, dp, C.c_double, C.c_double, dp]
+    L.oracle_nee_connect.argtypes = [P, C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, dp]
     L.oracle_hg_sample.argtypes = [C.c_double, dp, C.c_double, C.c_double, dp]
     L.oracle_hg_f.argtypes = [C.c_double, C.c_double]
     L.oracle_hg_f.restype = C.c_double
@@ -163,6 +165,23 @@ class OracleScene:
         g = np.zeros((len(pts), 3))
         self.L.oracle_field_eval(self.h, precision, len(pts), pts.reshape(-1), scaling, tau, n, g.reshape(-1))
         return n, g
+
+    def field_hessian(self, pts, scaling=1.0, tau=0.0):
+        """Hessian of n at points (FP64): (n, 3, 3)."""
+        pts = np.ascontiguousarray(np.asarray(pts, np.float64).reshape(-1, 3))
+        out = np.zeros((len(pts), 3, 3))
+        self.L.oracle_field_hessian(self.h, len(pts), pts.reshape(-1), scaling, tau, out.reshape(-1))
+        return out
+
+    def nee_connect(self, p1, v1, p2, scaling=1.0, tau=0.0):
+        """The reference's computefdfNEE (FP64) per ray: (p - p2 as (n, 3),
+        (dQ/dv1)^T as (n, 3, 3))."""
+        a, b, c = (np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1, 3)) for x in (p1, v1, p2))
+        err = np.zeros((len(a), 3))
+        jac = np.zeros((len(a), 3, 3))
+        self.L.oracle_nee_connect(self.h, len(a), a.reshape(-1), b.reshape(-1), c.reshape(-1), scaling, tau,
+                                  err.reshape(-1), jac.reshape(-1))
+        return err, jac
 
     def er_steps(self, p, v, h, nsteps, scaling=1.0, tau=0.0, precision=64):
         p = np.array(p, np.float64)

@@ -54,6 +54,7 @@ struct AdjState {
     int rlev, rleft, rfinal, racc;
     uint32_t rcnt0, rpat;
     int rphase, rseg, nsub, nmain, last_lam;
+    int slot;       // deferred re-trace: the photon's main-path slot, -1 none
     bool pending, trunc_main, trunc_sub;
     double F;
     uint32_t rsteps, nsubrev, nmainrev, dropped;
@@ -308,6 +309,7 @@ __device__ __forceinline__ void adj_photon_begin(AdjState &a) {
     a.nmain = 0;
     a.nsub = 0;
     a.last_lam = -1;
+    a.slot = -1;
     a.pending = false;
     a.trunc_main = false;
     a.trunc_sub = false;
@@ -344,6 +346,92 @@ __device__ __forceinline__ void adj_log_sub(Lane &L, f3 nrm) {
     ++a.nsub;
 }
 
+// ---------------------------------------------------------------------------
+// Deferred image adjoint.  The re-trace (M_ADJDEF) does not run backward
+// runs itself: a splat that touches G queues a sub-path job (its end state,
+// step pattern, face records, adjoint seeds and start point), and a photon
+// with a queued next-event sub-path stores its main-path log in a slot when
+// it ends.  k_adj_rev then runs the jobs (M_ADJSUB: each adds its vertex
+// adjoint to the slot's record) and afterwards the slots (M_ADJMAIN), every
+// lane of a wave running backwards, instead of the few reversing lanes of a
+// mostly marching wave.  A queue that fills sets qcnt[Q_OVER]; the host then
+// re-runs that chunk of photons in smaller pieces.
+
+__device__ __forceinline__ void adj_overflow(const RenderArgs &A) { atomicMax(A.qcnt + Q_OVER, 1ull); }
+
+__device__ __forceinline__ void copy_records(float *dst, const float *src, int n) {
+    float4 *d = reinterpret_cast<float4 *>(dst);
+    const float4 *s = reinterpret_cast<const float4 *>(src);
+    for (int i = 0; i < 4 * n; ++i) d[i] = s[i];
+}
+
+// Queue the current sub-path's backward run (the state sub_rev_start takes).
+__device__ __forceinline__ void adj_defer_sub(Lane &L, f3 lp, f3 lv, float mu, float ln, int phase) {
+    AdjState &a = *L.a;
+    const RenderArgs &A = L.A;
+    const Hot &h = L.h;
+    int slot = -1;
+    if (phase == RP_NEE) {
+        if (a.slot < 0) {
+            const unsigned long long sl = atomicAdd(A.qcnt + Q_SLOTS, 1ull);
+            if (sl >= A.slot_cap) {
+                adj_overflow(A);
+                return;
+            }
+            a.slot = (int)sl;
+        }
+        slot = a.slot;
+        a.last_lam = a.nmain - 1;
+    }
+    const unsigned long long j = atomicAdd(A.qcnt + Q_JOBS, 1ull);
+    if (j >= A.job_cap) {
+        adj_overflow(A);
+        return;
+    }
+    unsigned long long sb = 0;
+    if (a.nsub > 0) {
+        sb = atomicAdd(A.qcnt + Q_JOBREC, (unsigned long long)a.nsub);
+        if (sb + (unsigned long long)a.nsub > A.jobrec_cap) {
+            adj_overflow(A);
+            return;
+        }
+        copy_records(A.jobrec + sb * kRecFloats, a.log_sub, a.nsub);
+    }
+    float4 *r = reinterpret_cast<float4 *>(A.jobs + j * 32);
+    r[0] = make_float4(h.p.x, h.p.y, h.p.z, h.v.x);
+    r[1] = make_float4(h.v.y, h.v.z, __uint_as_float(h.fcnt0), __uint_as_float(h.fpat));
+    r[2] = make_float4(lp.x, lp.y, lp.z, mu);
+    r[3] = make_float4(lv.x, lv.y, lv.z, ln);
+    r[4] = make_float4(a.ldin.x, a.ldin.y, a.ldin.z, h.scaling);
+    r[5] = make_float4(L.c.mp.x, L.c.mp.y, L.c.mp.z, h.tau);
+    r[6] = make_float4(__int_as_float(slot), __int_as_float(a.nmain - 1), __int_as_float(phase),
+                       __int_as_float(a.nsub));
+    r[7] = make_float4(__uint_as_float((uint32_t)sb), __uint_as_float((uint32_t)(sb >> 32)), 0.f, 0.f);
+}
+
+// Photon end of the deferred re-trace: store the main-path log of a photon
+// with queued next-event sub-paths (slot layout: source, scaling; tau,
+// record count, last vertex with an adjoint, valid; record base).
+__device__ __forceinline__ void adj_store_slot(Lane &L) {
+    AdjState &a = *L.a;
+    const RenderArgs &A = L.A;
+    float4 *r = reinterpret_cast<float4 *>(A.slots + (size_t)a.slot * 16);
+    int valid = 0;
+    unsigned long long base = 0;
+    if (!a.trunc_main && a.nmain > 0) {
+        base = atomicAdd(A.qcnt + Q_SLOTREC, (unsigned long long)a.nmain);
+        if (base + (unsigned long long)a.nmain > A.slotrec_cap) {
+            adj_overflow(A);
+        } else {
+            copy_records(A.slotrec + base * kRecFloats, a.log_main, a.nmain);
+            valid = 1;
+        }
+    }
+    r[0] = make_float4(a.src.x, a.src.y, a.src.z, L.h.scaling);
+    r[1] = make_float4(L.h.tau, __int_as_float(a.nmain), __int_as_float(a.last_lam), __int_as_float(valid));
+    r[2] = make_float4(__uint_as_float((uint32_t)base), __uint_as_float((uint32_t)(base >> 32)), 0.f, 0.f);
+}
+
 // Start the backward run of the current sub-path from its sensor-face end
 // state (h.p, h.v): lp, lv = adjoint of the end state, mu = adjoint of the
 // travelled distance, ln = adjoint of n at the exit point.
@@ -372,7 +460,9 @@ __device__ __forceinline__ void sub_rev_start(Lane &L, f3 lp, f3 lv, float mu, f
 // NEE splat (scene.cpp:897-929) differentiated:
 //   val = W 2 pi exp(-sigma d) f_HG(din . dts) fore fw,
 //   fore = (refr . c) / (dts . c); fw (Fresnel transmission) is held fixed.
-template <int FIELD>
+// DEFER: queue the backward run (adj_defer_sub) and return false, so that the
+// forward run continues; else start it here.
+template <int FIELD, bool DEFER = false>
 __device__ __forceinline__ bool nee_adj_start(Lane &L, f3 ps, float Lv, float val, f3 din, f3 dts, float fore) {
     AdjState &a = *L.a;
     const DevScene &S = L.S;
@@ -398,12 +488,16 @@ __device__ __forceinline__ bool nee_adj_start(Lane &L, f3 ps, float Lv, float va
     exit_adj(S, L.h.p, L.h.v, L.h.n0, lps, ldts, lrefr, lp, lv, ln);
     const float ml = len(L.c.mv);
     a.ldin = (ldin - dot(din, ldin) * din) / ml;
+    if constexpr (DEFER) {
+        adj_defer_sub(L, lp, lv, -L.v->sigma_t * Wsum * val, ln, RP_NEE);
+        return false;
+    }
     sub_rev_start<FIELD>(L, lp, lv, -L.v->sigma_t * Wsum * val, ln, RP_NEE);
     return true;
 }
 
 // Direct splat (photon.cpp:53-80): val = W exp(-sigma d) fw.
-template <int FIELD>
+template <int FIELD, bool DEFER = false>
 __device__ __forceinline__ bool direct_adj_start(Lane &L, f3 ps, float Lv, float val) {
     AdjState &a = *L.a;
     const DevScene &S = L.S;
@@ -418,6 +512,10 @@ __device__ __forceinline__ bool direct_adj_start(Lane &L, f3 ps, float Lv, float
     f3 lp, lv;
     float ln;
     exit_adj(S, L.h.p, L.h.v, L.h.n0, lps, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f), lp, lv, ln);
+    if constexpr (DEFER) {
+        adj_defer_sub(L, lp, lv, -L.v->sigma_t * Wsum * val, ln, RP_DIRECT);
+        return false;
+    }
     sub_rev_start<FIELD>(L, lp, lv, -L.v->sigma_t * Wsum * val, ln, RP_DIRECT);
     return true;
 }

@@ -113,6 +113,11 @@ __device__ __forceinline__ void rev_end(Lane &L) {
         } else {
             direct_continue<FIELD>(L, 1);
         }
+    } else if constexpr (MODE == M_ADJSUB) {
+        // a queued sub-path: its vertex adjoint goes to the photon's stored
+        // main-path record (none when the log did not fit)
+        if (a.rphase == RP_NEE && a.log_main) rec_add_lambda(a.log_main, a.nmain - 1, a.lp + lw * h.g0, a.ldin);
+        h.state = ST_NEED;
     } else {
         photon_end(L);
     }
@@ -136,17 +141,26 @@ __device__ __forceinline__ void adj_flush_cfg(AdjState &a, const RenderArgs &A, 
 // loop plus the backward events (ST_REND, ST_MREV) and backward steps.
 // BATCH: photons of many point-source patterns (per-pattern amplitudes,
 // phases, photon budget, seed, objective weights, gradient and objective).
+// M_ADJDEF: the deferred image adjoint's re-trace: forward runs and logs
+// only, backward runs queued for k_adj_rev (no parameter derivatives here).
+// The spline re-trace is held to 256 registers (2 waves per SIMD, as the
+// spline render); it needs a few more, which spill.
 template <int FIELD, int MODE, bool BATCH = false>
-__global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(MODE == M_ADJDEF && FIELD == kSpline ? 2 : 1)))
+    k_adjoint(DevScene S, RenderArgs A) {
+    constexpr bool DEF = MODE == M_ADJDEF;
     __shared__ Cold cold[kBlock];
     __shared__ uint32_t cnt[C_N];
-    __shared__ float sacc[FIELD == kSources ? (2 * kMaxAdjSources + 1) * kBlock : (FIELD == kBessel ? 3 * kBlock : 1)];
-    __shared__ float vcell[FIELD == kSpline ? 64 * kBlock : 1];  // spline VJP cell sums
-    if (FIELD == kSpline)
+    __shared__ float sacc[DEF ? 1
+                              : (FIELD == kSources ? (2 * kMaxAdjSources + 1) * kBlock
+                                                   : (FIELD == kBessel ? 3 * kBlock : 1))];
+    __shared__ float vcell[FIELD == kSpline && !DEF ? 64 * kBlock : 1];  // spline VJP cell sums
+    if (FIELD == kSpline && !DEF)
         for (int t = 0; t < 64; ++t) vcell[t * kBlock + threadIdx.x] = 0.f;
     __shared__ Var vars[BATCH ? kBlock : 1];
     if (threadIdx.x < C_N) cnt[threadIdx.x] = 0u;
-    const int npar = FIELD == kSources ? 2 * S.n_src + 1 : (FIELD == kBessel ? 3 : 0);
+    const int npar = DEF ? 0 : (FIELD == kSources ? 2 * S.n_src + 1 : (FIELD == kBessel ? 3 : 0));
     for (int j = 0; j < npar; ++j) sacc[j * kBlock + threadIdx.x] = 0.f;
     __syncthreads();
     Var vsingle = var_from(S, A);
@@ -163,6 +177,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
     a.F = 0.0;
     a.rsteps = a.nsubrev = a.nmainrev = a.dropped = 0;
     a.pending = a.trunc_main = a.trunc_sub = false;
+    a.slot = -1;
     a.gL = 0.f;
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     a.log_main = A.log + gid * (size_t)(A.main_cap + A.sub_cap) * kRecFloats;
@@ -170,9 +185,9 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
     a.G.grid = A.grad;
     a.G.acc = sacc + threadIdx.x;
     a.G.stride = kBlock;
-    a.G.cell = FIELD == kSpline ? vcell + threadIdx.x : nullptr;
+    a.G.cell = FIELD == kSpline && !DEF ? vcell + threadIdx.x : nullptr;
     a.G.key = -1;
-    Lane L{h, cold[threadIdx.x], &v, S, A, cnt, &a, nullptr, nullptr};
+    Lane L{h, cold[threadIdx.x], &v, S, A, cnt, &a, nullptr, nullptr, nullptr, true};
     const int lane = __lane_id();
     while (true) {
         const unsigned long long need = __ballot(h.state == ST_NEED);
@@ -213,8 +228,15 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
         }
         if (h.state == ST_INIT) photon_init<FIELD, MODE>(L);
         if (h.state == ST_SEG_END) seg_end<FIELD, MODE>(L);
-        if (h.state == ST_REND) rev_end<FIELD, MODE>(L);
-        if (h.state == ST_MREV) main_rev_init<FIELD>(L);
+        if constexpr (DEF) {  // a photon with queued next-event sub-paths ended: store its main path
+            if (h.state == ST_NEED && a.slot >= 0) {
+                adj_store_slot(L);
+                a.slot = -1;
+            }
+        } else {
+            if (h.state == ST_REND) rev_end<FIELD, MODE>(L);
+            if (h.state == ST_MREV) main_rev_init<FIELD>(L);
+        }
         if (__ballot(h.state != ST_DONE) == 0ull) break;
         if (__ballot(h.state == ST_MARCH || h.state == ST_RMARCH) == 0ull) continue;
         const FieldP F = v.f;
@@ -224,7 +246,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
         for (int k = 0; k < A.spc; ++k) {
             if (m)
                 m = !march_step<FIELD, true, -1, true, -1, FIELD == kSpline>(h, sl, S, F, Lcut, &scell);
-            else if (r)
+            else if (!DEF && r)
                 r = rev_step<FIELD, FIELD == kSpline>(h, a, S, F, &scell);
         }
         h.steps_left = sl;
@@ -232,7 +254,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
         if (h.state == ST_RMARCH && !r) h.state = ST_REND;
     }
     if (BATCH) adj_flush_cfg(a, A, cur_cfg, npar);
-    if (FIELD == kSpline) spline_cell_flush(S, a.G);
+    if (FIELD == kSpline && !DEF) spline_cell_flush(S, a.G);
     unsigned long long st = h.steps;
     double fo = a.F;
     unsigned long long ac[A_N] = {a.rsteps, a.nsubrev, a.nmainrev, a.dropped};
@@ -253,6 +275,153 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(DevScene S, RenderArgs A) {
     if (threadIdx.x < C_N && threadIdx.x != C_STEPS && cnt[threadIdx.x])
         atomicAdd(A.counters + threadIdx.x, (unsigned long long)cnt[threadIdx.x]);
     for (int j = threadIdx.x; j < (BATCH ? 0 : npar); j += kBlock) {
+        float sum = 0.f;
+        for (int t = 0; t < kBlock; ++t) sum += sacc[j * kBlock + t];
+        if (sum != 0.f) atomicAdd(A.grad + j, sum);
+    }
+}
+
+// A queued sub-path (adj_defer_sub's job layout) as the lane's backward run.
+template <int FIELD>
+__device__ __forceinline__ void job_load(Lane &L, unsigned long long j) {
+    AdjState &a = *L.a;
+    Hot &h = L.h;
+    const RenderArgs &A = L.A;
+    const float4 *r = reinterpret_cast<const float4 *>(A.jobs + j * 32);
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6], r7 = r[7];
+    h.p = mk3(r0.x, r0.y, r0.z);
+    h.v = mk3(r0.w, r1.x, r1.y);
+    h.fcnt0 = __float_as_uint(r1.z);
+    h.fpat = __float_as_uint(r1.w);
+    h.scaling = r4.w;
+    h.tau = r5.w;
+    a.ldin = mk3(r4.x, r4.y, r4.z);
+    L.c.mp = mk3(r5.x, r5.y, r5.z);
+    const int slot = __float_as_int(r6.x);
+    a.nmain = __float_as_int(r6.y) + 1;
+    const int phase = __float_as_int(r6.z);
+    a.nsub = __float_as_int(r6.w);
+    const unsigned long long sb = (unsigned long long)__float_as_uint(r7.x) |
+                                  ((unsigned long long)__float_as_uint(r7.y) << 32);
+    a.log_sub = A.jobrec + sb * kRecFloats;
+    a.log_main = nullptr;
+    if (slot >= 0) {  // the photon's main-path log, when it was stored
+        const float4 s1 = reinterpret_cast<const float4 *>(A.slots + (size_t)slot * 16)[1];
+        const float4 s2 = reinterpret_cast<const float4 *>(A.slots + (size_t)slot * 16)[2];
+        if (__float_as_int(s1.w)) {
+            const unsigned long long base = (unsigned long long)__float_as_uint(s2.x) |
+                                            ((unsigned long long)__float_as_uint(s2.y) << 32);
+            a.log_main = A.slotrec + base * kRecFloats;
+        }
+    }
+    a.gL = 0.f;
+    sub_rev_start<FIELD>(L, mk3(r2.x, r2.y, r2.z), mk3(r3.x, r3.y, r3.z), r2.w, r3.w, phase);
+}
+
+// A stored main path (adj_store_slot's layout) as the lane's backward run;
+// false for a photon whose log was not stored.
+template <int FIELD>
+__device__ __forceinline__ bool slot_load(Lane &L, unsigned long long s) {
+    AdjState &a = *L.a;
+    Hot &h = L.h;
+    const RenderArgs &A = L.A;
+    const float4 *r = reinterpret_cast<const float4 *>(A.slots + s * 16);
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+    if (!__float_as_int(r1.w)) return false;
+    a.src = mk3(r0.x, r0.y, r0.z);
+    h.scaling = r0.w;
+    h.tau = r1.x;
+    a.nmain = __float_as_int(r1.y);
+    a.last_lam = __float_as_int(r1.z);
+    const unsigned long long base = (unsigned long long)__float_as_uint(r2.x) |
+                                    ((unsigned long long)__float_as_uint(r2.y) << 32);
+    a.log_main = A.slotrec + base * kRecFloats;
+    main_rev_init<FIELD>(L);
+    return true;
+}
+
+// The deferred image adjoint's backward kernel.  M_ADJSUB: the queued
+// next-event and direct sub-paths, each adding its vertex adjoint to the
+// photon's stored main-path log; M_ADJMAIN (launched after it): the stored
+// main paths.  Every lane runs backwards: items are fetched wave by wave from
+// the queue's own fetch counter, so a wave's lanes step together instead of
+// waiting on the few lanes of a marching wave that reverse.
+template <int FIELD, int MODE>
+__global__ void __launch_bounds__(kBlock) k_adj_rev(DevScene S, RenderArgs A) {
+    static_assert(MODE == M_ADJSUB || MODE == M_ADJMAIN, "backward modes only");
+    __shared__ uint32_t cnt[C_N];
+    __shared__ float sacc[FIELD == kSources ? (2 * kMaxAdjSources + 1) * kBlock : (FIELD == kBessel ? 3 * kBlock : 1)];
+    __shared__ float vcell[FIELD == kSpline ? 64 * kBlock : 1];  // spline VJP cell sums
+    if (FIELD == kSpline)
+        for (int t = 0; t < 64; ++t) vcell[t * kBlock + threadIdx.x] = 0.f;
+    if (threadIdx.x < C_N) cnt[threadIdx.x] = 0u;
+    const int npar = FIELD == kSources ? 2 * S.n_src + 1 : (FIELD == kBessel ? 3 : 0);
+    for (int j = 0; j < npar; ++j) sacc[j * kBlock + threadIdx.x] = 0.f;
+    __syncthreads();
+    const unsigned long long n_items =
+        MODE == M_ADJSUB ? min(A.qcnt[Q_JOBS], A.job_cap) : min(A.qcnt[Q_SLOTS], A.slot_cap);
+    unsigned long long *fetch = A.qcnt + (MODE == M_ADJSUB ? Q_WSUB : Q_WMAIN);
+    Var v = var_from(S, A);
+    Hot h;
+    h.steps = 0;
+    h.state = ST_NEED;
+    h.dead = false;
+    Cold c;  // only the sub-path start point (mp) is used
+    AdjState a;
+    SplineCell scell;
+    scell.key = -1;
+    a.F = 0.0;
+    a.rsteps = a.nsubrev = a.nmainrev = a.dropped = 0;
+    a.pending = a.trunc_main = a.trunc_sub = false;
+    a.slot = -1;
+    a.gL = 0.f;
+    a.log_main = a.log_sub = nullptr;
+    a.G.grid = A.grad;
+    a.G.acc = sacc + threadIdx.x;
+    a.G.stride = kBlock;
+    a.G.cell = FIELD == kSpline ? vcell + threadIdx.x : nullptr;
+    a.G.key = -1;
+    Lane L{h, c, &v, S, A, cnt, &a, nullptr, nullptr, nullptr, true};
+    const int lane = __lane_id();
+    const FieldP F = v.f;
+    while (true) {
+        const unsigned long long need = __ballot(h.state == ST_NEED);
+        if (need) {
+            const int leader = __ffsll((long long)need) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(fetch, (unsigned long long)__popcll(need));
+            base = __shfl(base, leader);
+            if (h.state == ST_NEED) {
+                const unsigned long long idx = base + __popcll(need & ((1ull << lane) - 1ull));
+                if (idx >= n_items)
+                    h.state = ST_DONE;
+                else if constexpr (MODE == M_ADJSUB)
+                    job_load<FIELD>(L, idx);
+                else if (!slot_load<FIELD>(L, idx))
+                    h.state = ST_NEED;
+            }
+        }
+        if (h.state == ST_REND) rev_end<FIELD, MODE>(L);
+        if (__ballot(h.state != ST_DONE) == 0ull) break;
+        if (__ballot(h.state == ST_RMARCH) == 0ull) continue;
+        bool r = h.state == ST_RMARCH;
+        for (int k = 0; k < A.spc; ++k)
+            if (r) r = rev_step<FIELD, FIELD == kSpline>(h, a, S, F, &scell);
+        if (h.state == ST_RMARCH && !r) h.state = ST_REND;
+    }
+    if (FIELD == kSpline) spline_cell_flush(S, a.G);
+    unsigned long long ac[A_N] = {a.rsteps, a.nsubrev, a.nmainrev, a.dropped};
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < A_N; ++k) ac[k] += __shfl_xor(ac[k], off);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < A_N; ++k)
+            if (ac[k]) atomicAdd(A.acnt + k, ac[k]);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < npar; j += kBlock) {
         float sum = 0.f;
         for (int t = 0; t < kBlock; ++t) sum += sacc[j * kBlock + t];
         if (sum != 0.f) atomicAdd(A.grad + j, sum);

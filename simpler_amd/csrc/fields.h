@@ -840,8 +840,12 @@ __device__ __forceinline__ void field_vjp<kSpline>(const DevScene &S, const Fiel
             for (int bq = 0; bq < 4; ++bq) {
                 const float Pw = (a * wy[bq] + by * dwy[bq]) * wz[c] + bz * wy[bq] * dwz[c];
                 const float Pd = bx * wy[bq] * wz[c];
+                // an LDS add without return (ds_add_f32): the lane owns its
+                // slots, so it needs neither the read nor its wait
 #pragma unroll
-                for (int q = 0; q < 4; ++q) G.cell[(c * 16 + bq * 4 + q) * G.stride] += Pw * wx[q] + Pd * dwx[q];
+                for (int q = 0; q < 4; ++q)
+                    __hip_atomic_fetch_add(G.cell + (c * 16 + bq * 4 + q) * G.stride, Pw * wx[q] + Pd * dwx[q],
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         return;
     }

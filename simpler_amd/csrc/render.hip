@@ -38,12 +38,21 @@ namespace simpler {
 
 enum { PH_DIRECT = 0, PH_NEE = 1, PH_FLIGHT = 2 };
 enum { ST_MARCH = 0, ST_SEG_END = 1, ST_NEED = 2, ST_INIT = 3, ST_DONE = 4, ST_RMARCH = 5, ST_REND = 6, ST_MREV = 7 };
-// kernel modes: image render, per-ray record, adjoint of an image objective,
-// adjoint of a per-ray objective
-// adjoint of a per-ray objective, per-ray record with tangents (forward mode)
-enum { M_RENDER = 0, M_RECORD = 1, M_ADJ = 2, M_ADJRAYS = 3, M_JVP = 4 };
+// kernel modes: image render, per-ray record, adjoint of an image objective
+// (backward runs inline), adjoint of a per-ray objective, per-ray record with
+// tangents (forward mode); the deferred image adjoint's three kernels: the
+// re-trace that queues the backward runs, the next-event / direct sub-path
+// backward runs, the main-path backward runs
+enum { M_RENDER = 0, M_RECORD = 1, M_ADJ = 2, M_ADJRAYS = 3, M_JVP = 4, M_ADJDEF = 5, M_ADJSUB = 6, M_ADJMAIN = 7 };
 __host__ __device__ constexpr bool per_ray(int m) { return m == M_RECORD || m == M_ADJRAYS || m == M_JVP; }
-__host__ __device__ constexpr bool is_adj(int m) { return m == M_ADJ || m == M_ADJRAYS; }
+__host__ __device__ constexpr bool is_adj(int m) {
+    return m == M_ADJ || m == M_ADJRAYS || m == M_ADJDEF || m == M_ADJSUB || m == M_ADJMAIN;
+}
+// the image-objective re-traces (inline or deferred backward runs)
+__host__ __device__ constexpr bool adj_image(int m) { return m == M_ADJ || m == M_ADJDEF; }
+// deferred image adjoint: queue counters (items, records, overflow flag) and
+// the backward kernels' fetch counters
+enum { Q_JOBS = 0, Q_JOBREC, Q_SLOTS, Q_SLOTREC, Q_OVER, Q_WSUB, Q_WMAIN, Q_N };
 enum { C_PHOT = 0, C_STEPS, C_SCAT, C_NEE, C_DIRECT, C_FACE, C_SEG, C_ABORT, C_N };
 constexpr uint32_t kMaxFaceEvents = 1u << 20;  // per march; see the abort counter
 constexpr int kBlock = 256;
@@ -81,6 +90,12 @@ struct RenderArgs {
     // directions, and the tangents of the recorded exit state
     const float *theta_dot, *v0_dot;
     float *tan_p, *tan_d, *tan_L;
+    // deferred image adjoint: sub-path jobs (8 float4) and their face
+    // records, main-path slots (4 float4) and their segment records (one
+    // 64-byte record each), queue counters (Q_*) and capacities
+    float *jobs, *jobrec, *slots, *slotrec;
+    unsigned long long *qcnt;
+    unsigned long long job_cap, jobrec_cap, slot_cap, slotrec_cap;
 };
 
 // Tangent of the photon state along a direction (forward mode): position,
@@ -160,6 +175,11 @@ struct Lane {
     float4 *mc;             // n, grad n at the saved main-path point (LDS; render kernels)
     PendSplat *ps;          // wave-aggregated splats: the lane's pending splat (else null)
     Tan *t;                 // forward-mode tangents (M_JVP kernels only)
+    // Adjoint kernels: a is set.  Code shared with the render kernels tests
+    // this flag, never the pointer: a null test of the address of the lane's
+    // adjoint state keeps the compiler from holding that state in registers
+    // (it stays in scratch memory).
+    bool adj = false;
 };
 
 __device__ __forceinline__ void count(Lane &L, int k) { atomicAdd(L.cnt + k, 1u); }
@@ -612,10 +632,12 @@ __device__ __forceinline__ bool exit_to_sensor(const DevScene &S, f3 &p, f3 d, f
 struct AdjState;
 __device__ __forceinline__ bool adj_main_pending(const AdjState *a);
 
-// End of a photon.  In the adjoint kernels a photon whose vertices received
-// adjoints first runs its main path backwards (ST_MREV).
+// End of a photon.  In the inline adjoint a photon whose vertices received
+// adjoints first runs its main path backwards (ST_MREV); the deferred
+// re-trace stores the photon's main path for the main-path kernel when it
+// sees the lane's photon end (k_adjoint).
 __device__ __forceinline__ void photon_end(Lane &L) {
-    L.h.state = (L.a && adj_main_pending(L.a)) ? ST_MREV : ST_NEED;
+    L.h.state = (L.adj && adj_main_pending(L.a)) ? ST_MREV : ST_NEED;
 }
 
 __device__ __forceinline__ void flight_start(Lane &L, float dist) {
@@ -697,7 +719,7 @@ __device__ __forceinline__ void vertex(Lane &L) {
         c.acc = 0.f;
         c.large = kLargeDist;
         c.faces = 0;
-        if (L.a) adj_sub_begin(*L.a);
+        if (L.adj) adj_sub_begin(*L.a);
         start_segment(L, kLargeDist);
         return;
     }
@@ -768,8 +790,8 @@ __device__ __forceinline__ void direct_done(Lane &L, int res) {
             h.L = Lv;
             count(L, C_DIRECT);
             const float val = L.v->weight * expf(-L.v->sigma_t * c.acc) * fw;
-            if constexpr (MODE == M_ADJ) {
-                if (direct_adj_start<FIELD>(L, p, Lv, val)) return;
+            if constexpr (adj_image(MODE)) {
+                if (direct_adj_start<FIELD, MODE == M_ADJDEF>(L, p, Lv, val)) return;
             } else {
                 splat(S, L.A, L.v->image, p, Lv, 0, val, L.ps);
             }
@@ -791,8 +813,8 @@ __device__ __forceinline__ void nee_done(Lane &L, int res) {
             const float val =
                 L.v->weight * (2.f * kPi) * expf(-L.v->sigma_t * c.acc) * hg_f(L.v->g, din, dts) * fore * fw;
             count(L, C_NEE);
-            if constexpr (MODE == M_ADJ) {
-                if (nee_adj_start<FIELD>(L, p, Lv, val, din, dts, fore)) return;
+            if constexpr (adj_image(MODE)) {
+                if (nee_adj_start<FIELD, MODE == M_ADJDEF>(L, p, Lv, val, din, dts, fore)) return;
             } else {
                 splat(S, L.A, L.v->image, p, Lv, c.depth, val, L.ps);
             }
@@ -823,7 +845,7 @@ __device__ __forceinline__ void seg_end(Lane &L) {
             const float mag = len(h.v);
             const f3 d1 = bsdf_sample(L, h.v / mag, nrm);
             if (dot(d1, nrm) < 0.f) {
-                if constexpr (MODE == M_ADJ) adj_log_main(L, K_FACE, nrm);
+                if constexpr (adj_image(MODE)) adj_log_main(L, K_FACE, nrm);
                 h.v = d1 * mag;
                 start_segment(L, c.large);
             } else {
@@ -837,7 +859,7 @@ __device__ __forceinline__ void seg_end(Lane &L) {
         } else {
             ++c.depth;
         }
-        if constexpr (MODE == M_ADJ) adj_log_main(L, K_DIST, mk3(0.f, 0.f, 0.f));
+        if constexpr (adj_image(MODE)) adj_log_main(L, K_DIST, mk3(0.f, 0.f, 0.f));
         vertex(L);
         return;
     }
@@ -961,7 +983,7 @@ __device__ __forceinline__ void photon_init(Lane &L) {
         }
     }
     if (S.use_direct || per_ray(MODE)) {
-        if (L.a) adj_sub_begin(*L.a);
+        if (L.adj) adj_sub_begin(*L.a);
         if (L.mc) *L.mc = make_float4(h.n0, h.g0.x, h.g0.y, h.g0.z);
         c.mp = h.p;
         c.mv = h.v;
@@ -1144,6 +1166,104 @@ __global__ void k_bessel_field_eval(DevScene S, int n, const float *pts, float s
     out_g[3 * i + 2] = g.z;
 }
 
+// State of the deterministic next-event march: position (+ compensation),
+// velocity as its start value v0 plus the kicks' sum dv (each kick, about
+// (h/2)|grad n| ~ 1e-7, is then kept to FP32 precision instead of being
+// rounded against |v| ~ 1.33), dp/dv1 and dv/dv1 by columns, and n, grad n,
+// H at p (the next step's start values).
+struct ConnState {
+    f3 p, pl, v0, dv;
+    f3 P[3], V[3];
+    float n;
+    f3 g;
+    Sym3 H;
+};
+
+// er_derivativestep (scene.cpp:351-359) with dV, d2V, d2Path (scene.h:253-272):
+//   v += h/2 grad n(p);  V += h/2 H(p) P;  p += h v / n(p);
+//   P += h [ -(1/n(p')^2) v (grad n(p') . P) + V / n(p') ];
+//   v += h/2 grad n(p'); V += h/2 H(p') P
+// (d2Path is taken at the new point p' with the half-kicked v and V, as the
+// reference does).
+template <int FIELD>
+__device__ __forceinline__ void conn_step(ConnState &s, float h, const DevScene &S, const FieldP &F, float scaling,
+                                          float tau) {
+    const float hh = 0.5f * h;
+    s.dv = s.dv + hh * s.g;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s.V[j] = s.V[j] + hh * symmul(s.H, s.P[j]);
+    const f3 v = s.v0 + s.dv;
+    const f3 dp = (h / s.n) * v;
+    comp_add(s.p.x, s.pl.x, dp.x);
+    comp_add(s.p.y, s.pl.y, dp.y);
+    comp_add(s.p.z, s.pl.z, dp.z);
+    field_eval_h<FIELD>(S, F, s.p, scaling, tau, s.n, s.g, s.H);
+    const float in = 1.0f / s.n;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s.P[j] = s.P[j] + h * ((-(dot(s.g, s.P[j]) * in * in)) * v + in * s.V[j]);
+    s.dv = s.dv + hh * s.g;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s.V[j] = s.V[j] + hh * symmul(s.H, s.P[j]);
+}
+
+// Scene::computefdfNEE (scene.cpp:512-574), one ray per thread: from p1
+// with v1 scaled to |v| = n(p1) (dv/dv1 = n(p1) (|v1|^2 I - v1 v1^T) / |v1|^3,
+// dp/dv1 = 0), march until x < x2, then bisect the last step nbis times
+// (a half step is kept when it stays at x > x2); the miss p - p2 and
+// (dQ/dv1)^T with dQ/dv1 = (I - t e_x^T / t_x) dp/dv1, t = v / n(p): the
+// crossing time's correction for the plane x = x2.
+template <int FIELD>
+__global__ void k_nee_connect(DevScene S, int n_rays, const float *p1, const float *v1, const float *p2,
+                              float scaling, float tau, int nbis, float *err, float *jac) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_rays) return;
+    const FieldP F = field_params(S);
+    ConnState s;
+    s.p = mk3(p1[3 * i], p1[3 * i + 1], p1[3 * i + 2]);
+    s.pl = mk3(0.f, 0.f, 0.f);
+    const f3 vi = mk3(v1[3 * i], v1[3 * i + 1], v1[3 * i + 2]);
+    const f3 q2 = mk3(p2[3 * i], p2[3 * i + 1], p2[3 * i + 2]);
+    field_eval_h<FIELD>(S, F, s.p, scaling, tau, s.n, s.g, s.H);
+    const float n1 = len(vi);
+    const float r = s.n;
+    const f3 e[3] = {mk3(1.f, 0.f, 0.f), mk3(0.f, 1.f, 0.f), mk3(0.f, 0.f, 1.f)};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        s.P[j] = mk3(0.f, 0.f, 0.f);
+        s.V[j] = (r / (n1 * n1 * n1)) * (n1 * n1 * e[j] - comp(vi, j) * vi);
+    }
+    s.v0 = (r / n1) * vi;
+    s.dv = mk3(0.f, 0.f, 0.f);
+    float h = S.h0;
+    ConnState old = s;
+    for (int it = 0; it < 100000; ++it) {
+        old = s;
+        conn_step<FIELD>(s, h, S, F, scaling, tau);
+        if (s.p.x < q2.x) {
+            while (nbis > 0) {
+                --nbis;
+                s = old;
+                h *= 0.5f;
+                conn_step<FIELD>(s, h, S, F, scaling, tau);
+                if (s.p.x > q2.x) old = s;
+            }
+            break;
+        }
+    }
+    const f3 p = s.p + s.pl;
+    const f3 t = (s.v0 + s.dv) / s.n;
+    err[3 * i] = p.x - q2.x;
+    err[3 * i + 1] = p.y - q2.y;
+    err[3 * i + 2] = p.z - q2.z;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const f3 Q = s.P[j] - (s.P[j].x / t.x) * t;
+        jac[9 * i + 3 * j] = Q.x;
+        jac[9 * i + 3 * j + 1] = Q.y;
+        jac[9 * i + 3 * j + 2] = Q.z;
+    }
+}
+
 }  // namespace simpler
 
 // ======================================================================
@@ -1206,6 +1326,12 @@ struct simpler_ctx {
     float *d_log = nullptr;  // adjoint segment logs (grown on demand)
     size_t log_bytes = 0;
     unsigned long long *d_adj = nullptr;  // adjoint objective (double) + counters
+    int adj_sched = 1;  // image adjoint: 0 backward runs inline, 1 deferred to k_adj_rev
+    int64_t adj_qcap = 1 << 23;  // deferred queues: sub-paths (and main paths, records) per chunk
+    int64_t adj_qalloc = 0;      // the capacity the queues were allocated for
+    int64_t adj_chunk = 1 << 22; // deferred adjoint: photons per chunk (adapted to the queues' fill)
+    float *d_jobs = nullptr, *d_jobrec = nullptr, *d_slots = nullptr, *d_slotrec = nullptr;
+    unsigned long long *d_q = nullptr;  // queue counters, then the per-chunk backups
     unsigned *d_pref = nullptr;           // prefilter: non-finite flag, minimum key
     hipEvent_t handoff = nullptr;         // orders a stream switch after the old stream's work
     std::string err;
@@ -1627,6 +1753,11 @@ int simpler_ctx_destroy(simpler_ctx *ctx) {
     (void)hipFree(ctx->d_scratch);
     (void)hipFree(ctx->d_log);
     (void)hipFree(ctx->d_adj);
+    (void)hipFree(ctx->d_jobs);
+    (void)hipFree(ctx->d_jobrec);
+    (void)hipFree(ctx->d_slots);
+    (void)hipFree(ctx->d_slotrec);
+    (void)hipFree(ctx->d_q);
     (void)hipFree(ctx->d_pref);
     (void)hipEventDestroy(ctx->handoff);
     (void)hipStreamDestroy(ctx->own);
@@ -1768,6 +1899,23 @@ int simpler_ctx_set_splat(simpler_ctx *ctx, int strategy) {
 int simpler_ctx_get_splat(simpler_ctx *ctx, int *strategy) {
     if (!ctx || !strategy) return SIMPLER_ERR_INVALID;
     *strategy = ctx->splat;
+    return SIMPLER_OK;
+}
+
+int simpler_ctx_set_adjoint_schedule(simpler_ctx *ctx, int schedule, int64_t queue_capacity) {
+    if (!ctx) return SIMPLER_ERR_INVALID;
+    if (schedule != 0 && schedule != 1) return fail(ctx, SIMPLER_ERR_INVALID, "adjoint schedule must be 0 or 1");
+    if (queue_capacity != 0 && (queue_capacity < 64 || queue_capacity > (int64_t(1) << 26)))
+        return fail(ctx, SIMPLER_ERR_INVALID, "adjoint queue capacity must be 0 or in [64, 2^26]");
+    ctx->adj_sched = schedule;
+    if (queue_capacity) ctx->adj_qcap = queue_capacity;
+    return SIMPLER_OK;
+}
+
+int simpler_ctx_get_adjoint_schedule(simpler_ctx *ctx, int *schedule, int64_t *queue_capacity) {
+    if (!ctx) return SIMPLER_ERR_INVALID;
+    if (schedule) *schedule = ctx->adj_sched;
+    if (queue_capacity) *queue_capacity = ctx->adj_qcap;
     return SIMPLER_OK;
 }
 
@@ -2259,6 +2407,51 @@ int simpler_jvp_rays(simpler_scene *sc, int64_t n_rays, uint64_t seed, uint64_t 
     return SIMPLER_OK;
 }
 
+int simpler_nee_connect(simpler_scene *sc, int64_t n_rays, const float *p1, const float *v1, const float *p2,
+                        float scaling, float tau, float *err, float *jac) {
+    if (!sc) return SIMPLER_ERR_INVALID;
+    simpler_ctx *ctx = sc->ctx;
+    int rc = scene_ready(sc);
+    if (rc) return rc;
+    const simpler_scene_desc &d = sc->desc;
+    if (n_rays < 0 || n_rays > 2147483647LL) return fail(ctx, SIMPLER_ERR_INVALID, "nee_connect: n_rays out of range");
+    if (n_rays > 0 && (!p1 || !v1 || !p2 || !err || !jac))
+        return fail(ctx, SIMPLER_ERR_INVALID, "nee_connect: null array");
+    if (d.field_kind == SIMPLER_FIELD_BESSEL && !(d.flags & SIMPLER_FLAG_GENERAL_AXIS)) {
+        // the Hessian is the general-axis form's, which the component form
+        // equals for the axis -x, and for m = 0 on +-x
+        const bool mx = d.axis_uz[0] == -1.f && d.axis_uz[1] == 0.f && d.axis_uz[2] == 0.f;
+        const bool px = d.axis_uz[0] == 1.f && d.axis_uz[1] == 0.f && d.axis_uz[2] == 0.f;
+        if (!(mx || (px && d.mode == 0)))
+            return fail(ctx, SIMPLER_ERR_INVALID,
+                        "nee_connect: the Bessel component form (general axis off) has a Hessian only for the axis "
+                        "-x, or +x at m = 0");
+    }
+    if (n_rays == 0) return SIMPLER_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const DevScene S = make_dev_scene(sc, 1, 0, true);
+    // scene.cpp:520: ceil(precision / log10 2) bisections of the last step
+    const int nbis = (int)std::ceil((double)d.precision / std::log10(2.0));
+    const int n = (int)n_rays;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    switch (d.field_kind) {
+        case SIMPLER_FIELD_BESSEL:
+            hipLaunchKernelGGL(k_nee_connect<kBessel>, grid, block, 0, ctx->cur, S, n, p1, v1, p2, scaling, tau, nbis,
+                               err, jac);
+            break;
+        case SIMPLER_FIELD_SOURCES:
+            hipLaunchKernelGGL(k_nee_connect<kSources>, grid, block, 0, ctx->cur, S, n, p1, v1, p2, scaling, tau, nbis,
+                               err, jac);
+            break;
+        default:
+            hipLaunchKernelGGL(k_nee_connect<kSpline>, grid, block, 0, ctx->cur, S, n, p1, v1, p2, scaling, tau, nbis,
+                               err, jac);
+            break;
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    return SIMPLER_OK;
+}
+
 int simpler_field_eval(simpler_scene *sc, int n_pts, const float *pts_dev, float scaling, float tau, float *out_n_dev,
                        float *out_g_dev) {
     if (!sc) return SIMPLER_ERR_INVALID;
@@ -2333,6 +2526,100 @@ static int adjoint_buffers(simpler_ctx *ctx) {
     return SIMPLER_OK;
 }
 
+// Deferred image adjoint: per chunk of photons the queues hold c = adj_qcap
+// sub-paths (128 B each) with c sub-path records, c main paths (64 B) with
+// 8 c main-path records (64 B each): 6 GiB at the default c = 2^23,
+// allocated on first use.  The 64^3 spline scene with scattering queues about
+// 0.6 sub-paths and 0.2 main paths per photon, with a few tens of records
+// each.  The chunk starts at 2^22 photons (a chunk drains like a launch of
+// its own, so larger is faster), halves when a chunk overflows and doubles
+// after a chunk that filled no queue past 40 %; it is kept by the context.
+constexpr int64_t kAdjChunkMin = 1 << 14, kAdjChunkMax = 1 << 24;
+constexpr int kQWords = 32;  // Q_N counters; objective / adjoint counters and event counters backed up at 16, 24
+
+static int adjoint_queues(simpler_ctx *ctx) {
+    if (ctx->d_q && ctx->adj_qalloc == ctx->adj_qcap) return SIMPLER_OK;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
+    float **bufs[4] = {&ctx->d_jobs, &ctx->d_jobrec, &ctx->d_slots, &ctx->d_slotrec};
+    for (float **b : bufs) {
+        (void)hipFree(*b);
+        *b = nullptr;
+    }
+    ctx->adj_qalloc = 0;
+    const size_t c = (size_t)ctx->adj_qcap;
+    HIP_TRY(ctx, hipMalloc(&ctx->d_jobs, c * 32 * sizeof(float)));
+    HIP_TRY(ctx, hipMalloc(&ctx->d_jobrec, c * kRecFloats * sizeof(float)));
+    HIP_TRY(ctx, hipMalloc(&ctx->d_slots, c * 16 * sizeof(float)));
+    HIP_TRY(ctx, hipMalloc(&ctx->d_slotrec, 8 * c * kRecFloats * sizeof(float)));
+    if (!ctx->d_q) HIP_TRY(ctx, hipMalloc(&ctx->d_q, kQWords * sizeof(unsigned long long)));
+    ctx->adj_qalloc = ctx->adj_qcap;
+    return SIMPLER_OK;
+}
+
+extern "C++" {
+template <int F>
+static void launch_adj_inline(simpler_ctx *ctx, dim3 grid, const DevScene &S, const RenderArgs &A) {
+    hipLaunchKernelGGL((k_adjoint<F, M_ADJ>), grid, dim3(kBlock), 0, ctx->cur, S, A);
+}
+
+// The image adjoint in chunks of photons: re-trace (queueing the backward
+// runs), then the queued sub-paths, then the stored main paths.  A chunk whose
+// queues overflow has its objective and counters restored and is re-traced
+// as a chunk half its size; at the smallest chunk it runs with the backward
+// runs inline.
+template <int F>
+static int adjoint_deferred(simpler_ctx *ctx, dim3 grid, const DevScene &S, RenderArgs A, int64_t n) {
+    A.jobs = ctx->d_jobs;
+    A.jobrec = ctx->d_jobrec;
+    A.slots = ctx->d_slots;
+    A.slotrec = ctx->d_slotrec;
+    A.qcnt = ctx->d_q;
+    A.job_cap = A.jobrec_cap = A.slot_cap = (unsigned long long)ctx->adj_qalloc;
+    A.slotrec_cap = 8ull * (unsigned long long)ctx->adj_qalloc;
+    const uint64_t off0 = A.offset;
+    const dim3 rgrid(std::max(1, ctx->num_cu * kAdjBlocksPerCU));
+    const double caps[4] = {(double)A.job_cap, (double)A.jobrec_cap, (double)A.slot_cap, (double)A.slotrec_cap};
+    int64_t start = 0;
+    while (start < n) {
+        const int64_t m = std::min<int64_t>(ctx->adj_chunk, n - start);
+        A.offset = off0 + (uint64_t)start;
+        A.n = (uint64_t)m;
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_q + 16, ctx->d_adj, 8 * sizeof(unsigned long long),
+                                    hipMemcpyDeviceToDevice, ctx->cur));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_q + 24, ctx->d_scratch + 8, 8 * sizeof(unsigned long long),
+                                    hipMemcpyDeviceToDevice, ctx->cur));
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, ctx->d_scratch, 1);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, ctx->d_q, Q_N);
+        hipLaunchKernelGGL((k_adjoint<F, M_ADJDEF>), grid, dim3(kBlock), 0, ctx->cur, S, A);
+        HIP_TRY(ctx, hipGetLastError());
+        unsigned long long q[Q_OVER + 1];
+        HIP_TRY(ctx, hipMemcpyAsync(q, ctx->d_q, sizeof(q), hipMemcpyDeviceToHost, ctx->cur));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
+        if (q[Q_OVER]) {
+            HIP_TRY(ctx, hipMemcpyAsync(ctx->d_adj, ctx->d_q + 16, 8 * sizeof(unsigned long long),
+                                        hipMemcpyDeviceToDevice, ctx->cur));
+            HIP_TRY(ctx, hipMemcpyAsync(ctx->d_scratch + 8, ctx->d_q + 24, 8 * sizeof(unsigned long long),
+                                        hipMemcpyDeviceToDevice, ctx->cur));
+            if (m > kAdjChunkMin) {  // re-trace the same photons in a smaller chunk
+                ctx->adj_chunk = std::max<int64_t>(kAdjChunkMin, m / 2);
+                continue;
+            }
+            hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, ctx->d_scratch, 1);
+            launch_adj_inline<F>(ctx, grid, S, A);
+        } else {
+            hipLaunchKernelGGL((k_adj_rev<F, M_ADJSUB>), rgrid, dim3(kBlock), 0, ctx->cur, S, A);
+            hipLaunchKernelGGL((k_adj_rev<F, M_ADJMAIN>), rgrid, dim3(kBlock), 0, ctx->cur, S, A);
+            double fill = 0.0;
+            for (int k = 0; k < 4; ++k) fill = std::max(fill, (double)q[k] / caps[k]);
+            if (m == ctx->adj_chunk && fill < 0.4 && ctx->adj_chunk < kAdjChunkMax) ctx->adj_chunk *= 2;
+        }
+        HIP_TRY(ctx, hipGetLastError());
+        start += m;
+    }
+    return SIMPLER_OK;
+}
+}  // extern "C++"
+
 static int adjoint_impl(simpler_scene *sc, int mode, int64_t n, int64_t n_total, uint64_t seed, uint64_t offset,
                         const float *G, int X, int Y, int Z, const float *up_p, const float *up_d, const float *up_L,
                         float *grad, int accumulate, double *obj_host, simpler_adjoint_stats *st) {
@@ -2402,13 +2689,23 @@ static int adjoint_impl(simpler_scene *sc, int mode, int64_t n, int64_t n_total,
     if (n > 0) {
         const dim3 grid(blocks), block(kBlock);
         const int fk = d.field_kind;
-        if (mode == M_ADJ) {
+        if (mode == M_ADJ && ctx->adj_sched == 1) {
+            rc = adjoint_queues(ctx);
+            if (rc) return rc;
             if (fk == SIMPLER_FIELD_SPLINE)
-                hipLaunchKernelGGL((k_adjoint<kSpline, M_ADJ>), grid, block, 0, ctx->cur, S, A);
+                rc = adjoint_deferred<kSpline>(ctx, grid, S, A, n);
             else if (fk == SIMPLER_FIELD_SOURCES)
-                hipLaunchKernelGGL((k_adjoint<kSources, M_ADJ>), grid, block, 0, ctx->cur, S, A);
+                rc = adjoint_deferred<kSources>(ctx, grid, S, A, n);
             else
-                hipLaunchKernelGGL((k_adjoint<kBessel, M_ADJ>), grid, block, 0, ctx->cur, S, A);
+                rc = adjoint_deferred<kBessel>(ctx, grid, S, A, n);
+            if (rc) return rc;
+        } else if (mode == M_ADJ) {
+            if (fk == SIMPLER_FIELD_SPLINE)
+                launch_adj_inline<kSpline>(ctx, grid, S, A);
+            else if (fk == SIMPLER_FIELD_SOURCES)
+                launch_adj_inline<kSources>(ctx, grid, S, A);
+            else
+                launch_adj_inline<kBessel>(ctx, grid, S, A);
         } else {
             if (fk == SIMPLER_FIELD_SPLINE)
                 hipLaunchKernelGGL((k_adjoint<kSpline, M_ADJRAYS>), grid, block, 0, ctx->cur, S, A);

@@ -66,6 +66,8 @@ def load_library():
     L.simpler_ctx_get_tuning.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.simpler_ctx_set_splat.argtypes = [P, C.c_int]
     L.simpler_ctx_get_splat.argtypes = [P, C.POINTER(C.c_int)]
+    L.simpler_ctx_set_adjoint_schedule.argtypes = [P, C.c_int, C.c_int64]
+    L.simpler_ctx_get_adjoint_schedule.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
     L.simpler_scene_create.argtypes = [P, C.POINTER(SceneDesc), fptr, C.c_int, C.c_int, PP]
     L.simpler_scene_destroy.argtypes = [P]
     L.simpler_scene_update.argtypes = [P, C.POINTER(SceneDesc)]
@@ -83,6 +85,7 @@ def load_library():
                                               C.c_int, C.c_int, C.c_int, C.POINTER(Stats)]
     L.simpler_trace_rays.argtypes = [P, C.c_int64, C.c_uint64, C.c_uint64, P, P, P, P]
     L.simpler_jvp_rays.argtypes = [P, C.c_int64, C.c_uint64, C.c_uint64, P, P, P, P, P, P, P, P, P]
+    L.simpler_nee_connect.argtypes = [P, C.c_int64, P, P, P, C.c_float, C.c_float, P, P]
     L.simpler_field_eval.argtypes = [P, C.c_int, P, C.c_float, C.c_float, P, P]
     L.simpler_read_stats.argtypes = [P, C.POINTER(Stats)]
     L.simpler_batch_create.argtypes = [P, C.c_int, C.POINTER(BatchCfg), C.c_int, C.c_int, C.c_int, PP]
@@ -101,9 +104,10 @@ def load_library():
     L.simpler_adjoint_rays.argtypes = [P, C.c_int64, C.c_uint64, C.c_uint64, P, P, P, P, C.c_int,
                                        C.POINTER(C.c_double), C.POINTER(AdjointStats)]
     for name in ("simpler_ctx_create", "simpler_ctx_destroy", "simpler_ctx_set_stream", "simpler_ctx_synchronize",
-                 "simpler_ctx_set_tuning", "simpler_ctx_get_tuning", "simpler_ctx_set_splat", "simpler_ctx_get_splat", "simpler_scene_create", "simpler_scene_destroy",
+                 "simpler_ctx_set_tuning", "simpler_ctx_get_tuning", "simpler_ctx_set_splat", "simpler_ctx_get_splat",
+                 "simpler_ctx_set_adjoint_schedule", "simpler_ctx_get_adjoint_schedule", "simpler_scene_create", "simpler_scene_destroy",
                  "simpler_scene_update", "simpler_scene_get_desc", "simpler_scene_set_sources",
-                 "simpler_scene_set_spline", "simpler_render", "simpler_render_host", "simpler_trace_rays", "simpler_jvp_rays",
+                 "simpler_scene_set_spline", "simpler_render", "simpler_render_host", "simpler_trace_rays", "simpler_jvp_rays", "simpler_nee_connect",
                  "simpler_field_eval", "simpler_read_stats", "simpler_render_host_noalloc", "simpler_batch_create", "simpler_batch_render",
                  "simpler_batch_graph_capture", "simpler_batch_graph_launch", "simpler_batch_destroy",
                  "simpler_adjoint_param_count", "simpler_adjoint_render", "simpler_adjoint_rays",
@@ -196,6 +200,17 @@ class Context:
         self.check(self.L.simpler_ctx_get_tuning(self.h, C.byref(b), C.byref(s), C.byref(c)))
         self.check(self.L.simpler_ctx_get_splat(self.h, C.byref(sp)))
         return dict(block_size=b.value, steps_per_chunk=s.value, blocks_per_cu=c.value, splat=sp.value)
+
+    def set_adjoint_schedule(self, schedule, queue_capacity=0):
+        """Image adjoint: 0 runs the backward runs inline in the re-trace, 1
+        defers them to full-wave backward kernels; queue_capacity sizes the
+        deferred queues per chunk of 2^20 photons (0 keeps it)."""
+        self.check(self.L.simpler_ctx_set_adjoint_schedule(self.h, int(schedule), int(queue_capacity)))
+
+    def get_adjoint_schedule(self):
+        s, c = C.c_int(), C.c_int64()
+        self.check(self.L.simpler_ctx_get_adjoint_schedule(self.h, C.byref(s), C.byref(c)))
+        return dict(schedule=s.value, queue_capacity=c.value)
 
     def read_stats(self):
         self._bind()
@@ -411,6 +426,24 @@ class Scene:
         self.ctx._bind()
         self.ctx.check(self.L.simpler_jvp_rays(self.h, n, seed, offset, ptr(td), ptr(vd), *[ptr(t) for t in out]))
         return tuple(out)
+
+    def nee_connect(self, p1, v1, p2, scaling=1.0, tau=0.0):
+        """Deterministic next-event connection (the reference's
+        computefdfNEE): march from p1 with velocity v1 (scaled to n(p1)) to
+        the plane x = p2.x; returns (p - p2 as (n, 3), (dQ/dv1)^T as
+        (n, 3, 3)).  p1, v1, p2: CUDA float32 tensors (n, 3)."""
+        import torch
+        a, b, c = (t.contiguous() for t in (p1, v1, p2))
+        n = a.shape[0]
+        for t in (a, b, c):
+            if t.dtype != torch.float32 or tuple(t.shape) != (n, 3) or not t.is_cuda:
+                raise SimplerError("p1, v1, p2 must be CUDA float32 tensors of shape (n, 3)")
+        err = torch.empty((n, 3), dtype=torch.float32, device=a.device)
+        jac = torch.empty((n, 3, 3), dtype=torch.float32, device=a.device)
+        ptr = lambda t: C.c_void_p(t.data_ptr())
+        self.ctx._bind()
+        self.ctx.check(self.L.simpler_nee_connect(self.h, n, ptr(a), ptr(b), ptr(c), scaling, tau, ptr(err), ptr(jac)))
+        return err, jac
 
     def autotune(self, n, res):
         """Pick block size, steps per chunk and blocks per CU for this scene

@@ -31,7 +31,7 @@ def gpu_threads():
 # GPU test files run in dependency order -- forward parity first, then the
 # layers built on it -- so that under -x a failure names the lowest layer.
 _GPU_ORDER = ["test_gpu_parity.py", "test_gpu_headline.py", "test_gpu_prefilter.py", "test_gpu_host_layer.py", "test_gpu_frontend.py",
-              "test_gpu_batch.py", "test_gpu_adjoint.py", "test_gpu_jvp.py"]
+              "test_gpu_batch.py", "test_gpu_adjoint.py", "test_gpu_jvp.py", "test_gpu_nee_connect.py"]
 
 
 def pytest_collection_modifyitems(config, items):
