@@ -182,11 +182,11 @@ __global__ void __launch_bounds__(kBlock)
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     a.log_main = A.log + gid * (size_t)(A.main_cap + A.sub_cap) * kRecFloats;
     a.log_sub = a.log_main + (size_t)A.main_cap * kRecFloats;
-    a.G.grid = A.grad;
+    a.G.grid = FIELD == kSpline && A.gx ? A.gx + (size_t)xcd_id() * A.gx_stride : A.grad;
     a.G.acc = sacc + threadIdx.x;
     a.G.stride = kBlock;
     a.G.cell = FIELD == kSpline && !DEF ? vcell + threadIdx.x : nullptr;
-    a.G.key = -1;
+    a.G.kx = -1;
     Lane L{h, cold[threadIdx.x], &v, S, A, cnt, &a, nullptr, nullptr, nullptr, true};
     const int lane = __lane_id();
     while (true) {
@@ -376,11 +376,11 @@ __global__ void __launch_bounds__(kBlock) k_adj_rev(DevScene S, RenderArgs A) {
     a.slot = -1;
     a.gL = 0.f;
     a.log_main = a.log_sub = nullptr;
-    a.G.grid = A.grad;
+    a.G.grid = FIELD == kSpline && A.gx ? A.gx + (size_t)xcd_id() * A.gx_stride : A.grad;
     a.G.acc = sacc + threadIdx.x;
     a.G.stride = kBlock;
     a.G.cell = FIELD == kSpline ? vcell + threadIdx.x : nullptr;
-    a.G.key = -1;
+    a.G.kx = -1;
     Lane L{h, c, &v, S, A, cnt, &a, nullptr, nullptr, nullptr, true};
     const int lane = __lane_id();
     const FieldP F = v.f;

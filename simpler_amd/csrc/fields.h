@@ -453,17 +453,29 @@ __device__ __forceinline__ f3 symmul(const Sym3 &H, f3 v) {
 }
 __device__ __forceinline__ Sym3 sym_zero() { return Sym3{0.f, 0.f, 0.f, 0.f, 0.f, 0.f}; }
 
+// The XCD (accelerator complex die) the calling wave runs on: hardware
+// register XCC_ID (id 20, bits 3:0).  A wave never moves between XCDs.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & (kXcds - 1); }
+
 struct GradSink {
-    float *grid;  // spline coefficient gradient (device)
+    // spline coefficient gradient (device): in the image and per-ray
+    // adjoints, the copy of the wave's XCD (the host sums the copies): a
+    // multi-XCD device performs device atomics beyond the L2, at the memory
+    // side, where atomics to one address serialise, and the reversed
+    // next-event sub-paths all converge on the cells in front of the aperture
+    float *grid;
     float *acc;   // source-parameter accumulators (LDS, lane-private)
     int stride;
-    // spline: the lane's sums for the 64 taps of one base cell (LDS, stride
-    // `stride`), added to grid when the lane's base cell changes (a
-    // reverse path spends about ten steps in a cell, so this cuts the
-    // device atomics about tenfold); key = the base tap's linear index, -1
-    // when empty; null for kernels without the buffer
+    // spline: the lane's sums for a 4x4x4 window of taps (LDS, 64 slots
+    // `stride` floats apart), tap (x, y, z) in slot (z&3)*16 + (y&3)*4 +
+    // (x&3); the window follows the lane's base cell, and only the sums of
+    // taps that leave it go to grid (16 of 64 for a move to a neighbouring
+    // cell; a reverse path moves to a neighbour about every ten steps).
+    // (kx, ky, kz): the window's base tap, kx = -1 when empty; cell null for
+    // kernels without the buffer
     float *cell;
-    int key;
+    int kx, ky, kz;
 };
 
 template <int FIELD>
@@ -789,29 +801,57 @@ __device__ __forceinline__ void field_eval_h<kSpline>(const DevScene &S, const F
     spline_eval_h<false>(S, p, n, g, H, nullptr);
 }
 
-// Adds the lane's cell sums to the coefficient gradient and empties them.
-__device__ __forceinline__ void spline_cell_flush(const DevScene &S, GradSink &G) {
-    if (G.key < 0) return;
-    const long NX = S.sN[0], NXY = (long)S.sN[0] * S.sN[1];
+// Moves the lane's tap window to base (nx, ny, nz); nx = -1 empties it.  The
+// sums of the taps that leave the window go to the gradient.  Per axis, the
+// old window's tap in slot residue r is X = kx + ((r - kx) & 3); it leaves
+// when X is outside [nx, nx + 4).
+__device__ __forceinline__ void spline_window_move(const DevScene &S, GradSink &G, int nx, int ny, int nz) {
+    if (G.kx >= 0) {
+        const long NX = S.sN[0], NXY = (long)S.sN[0] * S.sN[1];
+        int X[4], Y[4], Z[4];
+        unsigned lx = 0u, ly = 0u, lz = 0u;
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            float *row = G.grid + G.key + c * NXY + b * NX;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                float *slot = G.cell + (c * 16 + b * 4 + q) * G.stride;
-                unsafeAtomicAdd(row + q, *slot);
-                *slot = 0.f;
-            }
+        for (int r = 0; r < 4; ++r) {
+            X[r] = G.kx + ((r - G.kx) & 3);
+            Y[r] = G.ky + ((r - G.ky) & 3);
+            Z[r] = G.kz + ((r - G.kz) & 3);
+            if (nx < 0 || (unsigned)(X[r] - nx) > 3u) lx |= 1u << r;
+            if (nx < 0 || (unsigned)(Y[r] - ny) > 3u) ly |= 1u << r;
+            if (nx < 0 || (unsigned)(Z[r] - nz) > 3u) lz |= 1u << r;
         }
-    G.key = -1;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const bool plane = ((lz >> c) | (ly >> b)) & 1u;
+                float *row = G.grid + Z[c] * NXY + Y[b] * NX;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (!(plane || ((lx >> q) & 1u))) continue;
+                    float *slot = G.cell + (c * 16 + b * 4 + q) * G.stride;
+                    const float v = *slot;
+                    if (v != 0.f) unsafeAtomicAdd(row + X[q], v);
+                    *slot = 0.f;
+                }
+            }
+    }
+    G.kx = nx;
+    G.ky = ny;
+    G.kz = nz;
+}
+
+// Adds the lane's window sums to the coefficient gradient and empties them.
+__device__ __forceinline__ void spline_cell_flush(const DevScene &S, GradSink &G) {
+    spline_window_move(S, G, -1, -1, -1);
 }
 
 // coefficient (x, y, z) receives a wx wy wz + bx rx dwx wy wz + by ry wx dwy wz
-// + bz rz wx wy dwz.  Taps inside the grid go to the lane's cell sums (see
-// GradSink); near the faces, where mirror-wrapped taps may coincide, device
-// atomics add them directly.
+// + bz rz wx wy dwz.  Taps go to the lane's window sums (see GradSink); near
+// the faces the mirror-wrapped taps of a point still lie in one 4-wide window
+// per axis, clamped to the grid (coinciding taps share a slot, so their sums
+// add), which covers every point of the grid's box; points whose taps do not
+// fit one window (outside the box, or grids under 4 per axis) add them to
+// the gradient directly.
 template <>
 __device__ __forceinline__ void field_vjp<kSpline>(const DevScene &S, const FieldP &, f3 p, float, float, float a,
                                                    f3 b, GradSink &G) {
@@ -827,38 +867,52 @@ __device__ __forceinline__ void field_vjp<kSpline>(const DevScene &S, const Fiel
     const int ix = (int)fx - 1, iy = (int)fy - 1, iz = (int)fz - 1;
     const int NX = S.sN[0], NY = S.sN[1], NZ = S.sN[2];
     const float bx = b.x * S.sxres[0], by = b.y * S.sxres[1], bz = b.z * S.sxres[2];
-    if (G.cell && (unsigned)ix <= (unsigned)(NX - 4) && (unsigned)iy <= (unsigned)(NY - 4) &&
-        (unsigned)iz <= (unsigned)(NZ - 4)) {
-        const int key = (iz * NY + iy) * NX + ix;
-        if (key != G.key) {
-            spline_cell_flush(S, G);
-            G.key = key;
+    int xi[4], yi[4], zi[4];
+    const bool inside = (unsigned)ix <= (unsigned)(NX - 4) && (unsigned)iy <= (unsigned)(NY - 4) &&
+                        (unsigned)iz <= (unsigned)(NZ - 4);
+    if (inside) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            xi[q] = ix + q;
+            yi[q] = iy + q;
+            zi[q] = iz + q;
         }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            xi[q] = mirror_idx(ix + q, NX);
+            yi[q] = mirror_idx(iy + q, NY);
+            zi[q] = mirror_idx(iz + q, NZ);
+        }
+    }
+    const int kx = min(max(ix, 0), NX - 4), ky = min(max(iy, 0), NY - 4), kz = min(max(iz, 0), NZ - 4);
+    bool fit = G.cell != nullptr && kx >= 0 && ky >= 0 && kz >= 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        fit = fit && (unsigned)(xi[q] - kx) < 4u && (unsigned)(yi[q] - ky) < 4u && (unsigned)(zi[q] - kz) < 4u;
+    if (fit) {
+        if (kx != G.kx || ky != G.ky || kz != G.kz) spline_window_move(S, G, kx, ky, kz);
 #pragma unroll
         for (int c = 0; c < 4; ++c)
 #pragma unroll
             for (int bq = 0; bq < 4; ++bq) {
                 const float Pw = (a * wy[bq] + by * dwy[bq]) * wz[c] + bz * wy[bq] * dwz[c];
                 const float Pd = bx * wy[bq] * wz[c];
+                float *row = G.cell + (((zi[c] & 3) << 4) | ((yi[bq] & 3) << 2)) * G.stride;
                 // an LDS add without return (ds_add_f32): the lane owns its
                 // slots, so it needs neither the read nor its wait
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    __hip_atomic_fetch_add(G.cell + (c * 16 + bq * 4 + q) * G.stride, Pw * wx[q] + Pd * dwx[q],
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(row + (xi[q] & 3) * G.stride, Pw * wx[q] + Pd * dwx[q], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         return;
     }
-    int xi[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) xi[q] = mirror_idx(ix + q, NX);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const int zi = mirror_idx(iz + c, NZ);
 #pragma unroll
         for (int bq = 0; bq < 4; ++bq) {
-            const int yi = mirror_idx(iy + bq, NY);
-            float *row = G.grid + ((long)zi * NY + yi) * NX;
+            float *row = G.grid + ((long)zi[c] * NY + yi[bq]) * NX;
             const float Pw = (a * wy[bq] + by * dwy[bq]) * wz[c] + bz * wy[bq] * dwz[c];  // times wx
             const float Pd = bx * wy[bq] * wz[c];                                          // times dwx
 #pragma unroll

@@ -96,6 +96,10 @@ struct RenderArgs {
     float *jobs, *jobrec, *slots, *slotrec;
     unsigned long long *qcnt;
     unsigned long long job_cap, jobrec_cap, slot_cap, slotrec_cap;
+    // spline adjoint: one gradient copy per XCD (kXcds copies, gx_stride
+    // floats apart), summed into grad by k_gx_reduce
+    float *gx;
+    size_t gx_stride;
 };
 
 // Tangent of the photon state along a direction (forward mode): position,
@@ -1132,6 +1136,19 @@ k_render(DevScene S, RenderArgs A) {
 __global__ void k_fill_f32(float *p, size_t n, float v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
+// grad += the sum of the per-XCD gradient copies, which are left zero.
+__global__ void k_gx_reduce(float *grad, float *gx, size_t n, size_t stride) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float sum = 0.f;
+#pragma unroll
+        for (int k = 0; k < kXcds; ++k) {
+            sum += gx[k * stride + i];
+            gx[k * stride + i] = 0.f;
+        }
+        if (sum != 0.f) grad[i] += sum;
+    }
+}
+
 __global__ void k_fill_u64(unsigned long long *p, int n) {
     const int i = threadIdx.x;
     if (i < n) p[i] = 0ull;
@@ -1330,6 +1347,8 @@ struct simpler_ctx {
     int64_t adj_qcap = 1 << 23;  // deferred queues: sub-paths (and main paths, records) per chunk
     int64_t adj_qalloc = 0;      // the capacity the queues were allocated for
     int64_t adj_chunk = 1 << 22; // deferred adjoint: photons per chunk (adapted to the queues' fill)
+    float *d_gx = nullptr;       // spline adjoint: per-XCD gradient copies (kept zero between calls)
+    size_t gx_floats = 0;        // floats per copy
     float *d_jobs = nullptr, *d_jobrec = nullptr, *d_slots = nullptr, *d_slotrec = nullptr;
     unsigned long long *d_q = nullptr;  // queue counters, then the per-chunk backups
     unsigned *d_pref = nullptr;           // prefilter: non-finite flag, minimum key
@@ -1758,6 +1777,7 @@ int simpler_ctx_destroy(simpler_ctx *ctx) {
     (void)hipFree(ctx->d_slots);
     (void)hipFree(ctx->d_slotrec);
     (void)hipFree(ctx->d_q);
+    (void)hipFree(ctx->d_gx);
     (void)hipFree(ctx->d_pref);
     (void)hipEventDestroy(ctx->handoff);
     (void)hipStreamDestroy(ctx->own);
@@ -2556,6 +2576,27 @@ static int adjoint_queues(simpler_ctx *ctx) {
     return SIMPLER_OK;
 }
 
+// Per-XCD gradient copies for a spline adjoint of n parameters (A.gx),
+// allocated on first use, zeroed then and left zero by k_gx_reduce.
+static int adjoint_gx(simpler_ctx *ctx, int64_t n, RenderArgs &A) {
+    const size_t stride = ((size_t)n + 63) & ~(size_t)63;
+    if (ctx->gx_floats < stride) {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
+        (void)hipFree(ctx->d_gx);
+        ctx->d_gx = nullptr;
+        ctx->gx_floats = 0;
+        HIP_TRY(ctx, hipMalloc(&ctx->d_gx, kXcds * stride * sizeof(float)));
+        const size_t total = kXcds * stride;
+        hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)std::min<size_t>((total + 255) / 256, 8192)), dim3(256), 0,
+                           ctx->cur, ctx->d_gx, total, 0.0f);
+        HIP_TRY(ctx, hipGetLastError());
+        ctx->gx_floats = stride;
+    }
+    A.gx = ctx->d_gx;
+    A.gx_stride = ctx->gx_floats;
+    return SIMPLER_OK;
+}
+
 extern "C++" {
 template <int F>
 static void launch_adj_inline(simpler_ctx *ctx, dim3 grid, const DevScene &S, const RenderArgs &A) {
@@ -2686,6 +2727,10 @@ static int adjoint_impl(simpler_scene *sc, int mode, int64_t n, int64_t n_total,
         const int fb = (int)std::min<int64_t>((npar + 255) / 256, 8192);
         hipLaunchKernelGGL(k_fill_f32, dim3(fb), dim3(256), 0, ctx->cur, grad, (size_t)npar, 0.0f);
     }
+    if (d.field_kind == SIMPLER_FIELD_SPLINE && n > 0) {
+        rc = adjoint_gx(ctx, npar, A);
+        if (rc) return rc;
+    }
     if (n > 0) {
         const dim3 grid(blocks), block(kBlock);
         const int fk = d.field_kind;
@@ -2714,6 +2759,10 @@ static int adjoint_impl(simpler_scene *sc, int mode, int64_t n, int64_t n_total,
             else
                 hipLaunchKernelGGL((k_adjoint<kBessel, M_ADJRAYS>), grid, block, 0, ctx->cur, S, A);
         }
+    }
+    if (A.gx) {
+        const unsigned gb = (unsigned)std::min<int64_t>((npar + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_gx_reduce, dim3(gb), dim3(256), 0, ctx->cur, grad, A.gx, (size_t)npar, A.gx_stride);
     }
     HIP_TRY(ctx, hipGetLastError());
     if (obj_host || st) {
