@@ -213,6 +213,7 @@ def adjoint_bench(args):
     from simpler_amd import engine, scenes
 
     ctx = engine.Context(0)
+    ctx.set_adjoint_schedule(args.adjoint_schedule)
     d, tex, res, (coeff, dims, lo, hi) = scenes.spline_cube_ready(n=64)
     sc = engine.Scene(ctx, d, tex, spline=(coeff, dims, lo, hi))
     G = torch.tensor(scenes.aperture_weights(res), device="cuda")
@@ -500,7 +501,8 @@ def copy_calibration(nbytes=1 << 30, reps=3):
 def profile_workload(args):
     """One launch of a named workload, for hardware-counter runs:
     'spline' = the differentiable-spline forward render, 'adjoint' = its
-    path-replay adjoint (2^18 photons each)."""
+    path-replay adjoint (--profile-photons photons, under
+    --adjoint-schedule)."""
     import torch
 
     from simpler_amd import engine, scenes
@@ -508,7 +510,8 @@ def profile_workload(args):
     ctx = engine.Context(0)
     d, tex, res, spl = scenes.spline_cube_ready(n=64)
     sc = engine.Scene(ctx, d, tex, spline=spl)
-    n = 1 << 18
+    n = args.profile_photons
+    ctx.set_adjoint_schedule(args.adjoint_schedule)
     if args.profile == "spline":
         _, st = sc.render(n, res, seed=11)
     else:
@@ -539,6 +542,9 @@ def main():
     ap.add_argument("--prefilter-reps", type=int, default=5)
     ap.add_argument("--profile", choices=["spline", "adjoint", "prefilter"], default="",
                     help="only run one launch of this workload (for hardware-counter runs)")
+    ap.add_argument("--profile-photons", type=int, default=1 << 18)
+    ap.add_argument("--adjoint-schedule", type=int, default=1, choices=[0, 1],
+                    help="image adjoint: 0 backward runs inline, 1 deferred to full-wave kernels")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling rocprof byte counters)")
     ap.add_argument("--no-batch", action="store_true")

@@ -1115,7 +1115,9 @@ k_render(DevScene S, RenderArgs A) {
         bool m = h.state == ST_MARCH;
         uint32_t sl = h.steps_left;
         for (int k = 0; k < A.spc; ++k) {
-            wave_steps += __popcll(__ballot(m));  // one ER step per marching lane
+            const int na = __popcll(__ballot(m));
+            if (na == 0) break;  // every lane of the chunk has ended its segment
+            wave_steps += na;    // one ER step per marching lane
             if constexpr (MODE == M_JVP) {
                 if (m) m = !march_step_jvp<FIELD>(h, tan, sl, S, F, Lcut, A.theta_dot);
             } else {
