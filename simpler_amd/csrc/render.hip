@@ -100,6 +100,11 @@ struct RenderArgs {
     // floats apart), summed into grad by k_gx_reduce
     float *gx;
     size_t gx_stride;
+    // spline image adjoint: the objective's support (pixel box over all
+    // planes: min x, max x, min y, max y; max x < 0 when it is empty) and the
+    // largest coefficient difference along each axis (float bits), for
+    // nee_skip; null: every next-event sub-path is traced
+    const int *nee_cut;
 };
 
 // Tangent of the photon state along a direction (forward mode): position,
@@ -694,6 +699,43 @@ __device__ __forceinline__ void restore_main(Lane &L, float Lval) {
 #include "adjoint.h"
 namespace simpler {
 
+// Image adjoint of a spline field: can the next-event sub-path that starts
+// at the lane's vertex in direction d (unit) reach a pixel with a nonzero
+// objective weight?  Its straight line reaches the sensor face (x = blockL.x,
+// where the host has checked the camera sits, looking along -x) at q0 after a
+// length t.  The field bends the path by at most kappa s^2 / 2 over a length
+// s, with kappa >= |grad n| / n (n >= 1): |dn/dx_a| <= res_a max|c_{i+1} -
+// c_i| along a, the cubic B-spline's derivative being a convex combination
+// of coefficient differences.  If q0 is more than that bound inside the
+// face's edges, the path cannot meet a side face first (the distance to a
+// side face is linear along the line and the bound convex) and leaves
+// through the sensor face within the bound of q0; if every pixel its splat
+// can then touch (two taps per axis) lies outside the objective's support,
+// the sub-path adds nothing to F or to the gradient.  The margin (1e-5 m and
+// a pixel) covers the march's stop past the face (under 1e-9 m), the inside
+// test's eps and rounding.
+__device__ __forceinline__ bool nee_skip(const Lane &L, f3 d) {
+    const int *cut = L.A.nee_cut;
+    if (!cut || !(d.x < -0.05f)) return false;
+    const DevScene &S = L.S;
+    const f3 p = L.h.p;
+    const float t = (p.x - S.inL.x) / -d.x;  // the march leaves at the inside test's bound
+    const float gx = __int_as_float(cut[4]) * S.sxres[0], gy = __int_as_float(cut[5]) * S.sxres[1],
+                gz = __int_as_float(cut[6]) * S.sxres[2];
+    const float kappa = 1.01f * sqrtf(gx * gx + gy * gy + gz * gz);
+    const float s = 1.05f * t + 1e-5f;
+    const float dev = 0.5f * kappa * s * s + 1e-5f;
+    const f3 q = p + t * d;
+    if (!(q.y > S.blockL.y + dev && q.y < S.blockR.y - dev && q.z > S.blockL.z + dev && q.z < S.blockR.z - dev))
+        return false;
+    const float x = (dot(S.cam_h, q) - S.cam_o.y) / S.view_plane[0] * (float)L.A.X + 0.5f * (float)L.A.X;
+    const float y = (dot(S.cam_v, q) - S.cam_o.z) / S.view_plane[1] * (float)L.A.Y + 0.5f * (float)L.A.Y;
+    const float mx = dev * (float)L.A.X / S.view_plane[0] + 1.f, my = dev * (float)L.A.Y / S.view_plane[1] + 1.f;
+    if (cut[1] < 0) return true;  // no pixel has a nonzero weight
+    return x + mx < (float)(cut[0] - 1) || x - mx >= (float)(cut[1] + 1) || y + my < (float)(cut[2] - 1) ||
+           y - my >= (float)(cut[3] + 1);
+}
+
 // photon.cpp:111-116 loop head + scene.cpp:866-893 (NEE start)
 __device__ __forceinline__ void vertex(Lane &L) {
     const DevScene &S = L.S;
@@ -723,6 +765,11 @@ __device__ __forceinline__ void vertex(Lane &L) {
         c.acc = 0.f;
         c.large = kLargeDist;
         c.faces = 0;
+        if (L.adj && nee_skip(L, mk3(-z, ca * tmp, sa * tmp))) {  // ends as a dead march: no splat
+            h.dead = true;
+            h.state = ST_SEG_END;
+            return;
+        }
         if (L.adj) adj_sub_begin(*L.a);
         start_segment(L, kLargeDist);
         return;
@@ -1138,6 +1185,66 @@ k_render(DevScene S, RenderArgs A) {
 __global__ void k_fill_f32(float *p, size_t n, float v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
+// The support of objective weights G (Z planes of X x Y): the pixel box of
+// its nonzero weights over all planes, into box = {min x, max x, min y,
+// max y}, which must hold {INT_MAX, -1, INT_MAX, -1} before.
+__global__ void k_gbox(const float *G, int X, int Y, int Z, int *box) {
+    int x0 = 0x7fffffff, x1 = -1, y0 = 0x7fffffff, y1 = -1;
+    const size_t n = (size_t)X * Y * Z;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (G[i] != 0.f) {
+            const int x = (int)(i % (size_t)X), y = (int)((i / (size_t)X) % (size_t)Y);
+            x0 = min(x0, x);
+            x1 = max(x1, x);
+            y0 = min(y0, y);
+            y1 = max(y1, y);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        x0 = min(x0, __shfl_xor(x0, off));
+        x1 = max(x1, __shfl_xor(x1, off));
+        y0 = min(y0, __shfl_xor(y0, off));
+        y1 = max(y1, __shfl_xor(y1, off));
+    }
+    if (__lane_id() == 0 && x1 >= 0) {
+        atomicMin(box, x0);
+        atomicMax(box + 1, x1);
+        atomicMin(box + 2, y0);
+        atomicMax(box + 3, y1);
+    }
+}
+
+// The largest coefficient difference along each axis of an N0 x N1 x N2
+// grid (x fastest), as the bits of non-negative floats (which order as
+// unsigned integers) into dmax[3], which must hold zeros before.
+__global__ void k_spline_dmax(const float *c, int N0, int N1, int N2, unsigned *dmax) {
+    float m0 = 0.f, m1 = 0.f, m2 = 0.f;
+    const size_t row = (size_t)N0, slab = (size_t)N0 * N1, n = slab * N2;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % row), y = (int)((i / row) % (size_t)N1), z = (int)(i / slab);
+        const float v = c[i];
+        if (x + 1 < N0) m0 = fmaxf(m0, fabsf(c[i + 1] - v));
+        if (y + 1 < N1) m1 = fmaxf(m1, fabsf(c[i + row] - v));
+        if (z + 1 < N2) m2 = fmaxf(m2, fabsf(c[i + slab] - v));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        m0 = fmaxf(m0, __shfl_xor(m0, off));
+        m1 = fmaxf(m1, __shfl_xor(m1, off));
+        m2 = fmaxf(m2, __shfl_xor(m2, off));
+    }
+    if (__lane_id() == 0) {
+        atomicMax(dmax, __float_as_uint(m0));
+        atomicMax(dmax + 1, __float_as_uint(m1));
+        atomicMax(dmax + 2, __float_as_uint(m2));
+    }
+}
+
+// Initial values of the k_gbox / k_spline_dmax accumulators.
+__global__ void k_cut_init(int *cut) {
+    const int init[8] = {0x7fffffff, -1, 0x7fffffff, -1, 0, 0, 0, 0};
+    if (threadIdx.x < 8) cut[threadIdx.x] = init[threadIdx.x];
+}
+
 // grad += the sum of the per-XCD gradient copies, which are left zero.
 __global__ void k_gx_reduce(float *grad, float *gx, size_t n, size_t stride) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -1350,6 +1457,7 @@ struct simpler_ctx {
     int64_t adj_qalloc = 0;      // the capacity the queues were allocated for
     int64_t adj_chunk = 1 << 22; // deferred adjoint: photons per chunk (adapted to the queues' fill)
     float *d_gx = nullptr;       // spline adjoint: per-XCD gradient copies (kept zero between calls)
+    int *d_cut = nullptr;        // spline image adjoint: objective support and coefficient differences
     size_t gx_floats = 0;        // floats per copy
     float *d_jobs = nullptr, *d_jobrec = nullptr, *d_slots = nullptr, *d_slotrec = nullptr;
     unsigned long long *d_q = nullptr;  // queue counters, then the per-chunk backups
@@ -1780,6 +1888,7 @@ int simpler_ctx_destroy(simpler_ctx *ctx) {
     (void)hipFree(ctx->d_slotrec);
     (void)hipFree(ctx->d_q);
     (void)hipFree(ctx->d_gx);
+    (void)hipFree(ctx->d_cut);
     (void)hipFree(ctx->d_pref);
     (void)hipEventDestroy(ctx->handoff);
     (void)hipStreamDestroy(ctx->own);
@@ -2732,6 +2841,23 @@ static int adjoint_impl(simpler_scene *sc, int mode, int64_t n, int64_t n_total,
     if (d.field_kind == SIMPLER_FIELD_SPLINE && n > 0) {
         rc = adjoint_gx(ctx, npar, A);
         if (rc) return rc;
+        // next-event sub-paths that cannot reach the objective are not
+        // traced (nee_skip): spline fields, the camera on the block's -x face
+        // looking along -x (where the next-event hemisphere points)
+        const bool cam_on_face = d.view_dir[0] < 0.f && d.view_dir[1] == 0.f && d.view_dir[2] == 0.f &&
+                                 d.view_origin[0] == d.block_l[0];
+        if (mode == M_ADJ && cam_on_face) {
+            if (!ctx->d_cut) HIP_TRY(ctx, hipMalloc(&ctx->d_cut, 8 * sizeof(int)));
+            hipLaunchKernelGGL(k_cut_init, dim3(1), dim3(64), 0, ctx->cur, ctx->d_cut);
+            const size_t ng = (size_t)X * Y * Z, nc = (size_t)sc->sN[0] * sc->sN[1] * sc->sN[2];
+            hipLaunchKernelGGL(k_gbox, dim3((unsigned)std::min<size_t>((ng + 255) / 256, 4096)), dim3(256), 0,
+                               ctx->cur, G, X, Y, Z, ctx->d_cut);
+            hipLaunchKernelGGL(k_spline_dmax, dim3((unsigned)std::min<size_t>((nc + 255) / 256, 4096)), dim3(256), 0,
+                               ctx->cur, sc->d_spline, sc->sN[0], sc->sN[1], sc->sN[2],
+                               reinterpret_cast<unsigned *>(ctx->d_cut + 4));
+            HIP_TRY(ctx, hipGetLastError());
+            A.nee_cut = ctx->d_cut;
+        }
     }
     if (n > 0) {
         const dim3 grid(blocks), block(kBlock);
