@@ -55,6 +55,7 @@ struct AdjState {
     uint32_t rcnt0, rpat;
     int rphase, rseg, nsub, nmain, last_lam;
     int slot;       // deferred re-trace: the photon's main-path slot, -1 none
+    int cfg;        // batched kernels: the lane's configuration
     bool pending, trunc_main, trunc_sub;
     double F;
     uint32_t rsteps, nsubrev, nmainrev, dropped;
@@ -406,12 +407,14 @@ __device__ __forceinline__ void adj_defer_sub(Lane &L, f3 lp, f3 lv, float mu, f
     r[5] = make_float4(L.c.mp.x, L.c.mp.y, L.c.mp.z, h.tau);
     r[6] = make_float4(__int_as_float(slot), __int_as_float(a.nmain - 1), __int_as_float(phase),
                        __int_as_float(a.nsub));
-    r[7] = make_float4(__uint_as_float((uint32_t)sb), __uint_as_float((uint32_t)(sb >> 32)), 0.f, 0.f);
+    r[7] = make_float4(__uint_as_float((uint32_t)sb), __uint_as_float((uint32_t)(sb >> 32)), __int_as_float(a.cfg),
+                       0.f);
 }
 
 // Photon end of the deferred re-trace: store the main-path log of a photon
 // with queued next-event sub-paths (slot layout: source, scaling; tau,
-// record count, last vertex with an adjoint, valid; record base).
+// record count, last vertex with an adjoint, valid; record base,
+// configuration).
 __device__ __forceinline__ void adj_store_slot(Lane &L) {
     AdjState &a = *L.a;
     const RenderArgs &A = L.A;
@@ -429,7 +432,8 @@ __device__ __forceinline__ void adj_store_slot(Lane &L) {
     }
     r[0] = make_float4(a.src.x, a.src.y, a.src.z, L.h.scaling);
     r[1] = make_float4(L.h.tau, __int_as_float(a.nmain), __int_as_float(a.last_lam), __int_as_float(valid));
-    r[2] = make_float4(__uint_as_float((uint32_t)base), __uint_as_float((uint32_t)(base >> 32)), 0.f, 0.f);
+    r[2] = make_float4(__uint_as_float((uint32_t)base), __uint_as_float((uint32_t)(base >> 32)), __int_as_float(a.cfg),
+                       0.f);
 }
 
 // Start the backward run of the current sub-path from its sensor-face end

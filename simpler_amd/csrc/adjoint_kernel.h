@@ -178,6 +178,7 @@ __global__ void __launch_bounds__(kBlock)
     a.rsteps = a.nsubrev = a.nmainrev = a.dropped = 0;
     a.pending = a.trunc_main = a.trunc_sub = false;
     a.slot = -1;
+    a.cfg = 0;
     a.gL = 0.f;
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     a.log_main = A.log + gid * (size_t)(A.main_cap + A.sub_cap) * kRecFloats;
@@ -202,18 +203,22 @@ __global__ void __launch_bounds__(kBlock)
                 if (idx < A.n) {
                     uint64_t pid = A.offset + idx;
                     if (BATCH) {
-                        int lo = 0, hi = A.n_cfg;  // largest c with prefix[c] <= idx
+                        // the photon's index over all configurations (A.offset:
+                        // the start of a deferred chunk)
+                        const unsigned long long gi = A.offset + idx;
+                        int lo = 0, hi = A.n_cfg;  // largest c with prefix[c] <= gi
                         while (hi - lo > 1) {
                             const int mid = (lo + hi) >> 1;
-                            if (A.prefix[mid] <= idx)
+                            if (A.prefix[mid] <= gi)
                                 lo = mid;
                             else
                                 hi = mid;
                         }
-                        pid = idx - A.prefix[lo];
+                        pid = gi - A.prefix[lo];
                         if (lo != cur_cfg) {
                             adj_flush_cfg(a, A, cur_cfg, npar);
                             cur_cfg = lo;
+                            a.cfg = lo;
                             v = A.cfgs[lo];
                             v.image = const_cast<float *>(A.G) + (size_t)lo * A.X * A.Y * A.Z;
                         }
@@ -281,6 +286,13 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// The configuration of a queued job or stored slot (batched adjoint).
+template <int MODE>
+__device__ __forceinline__ int item_cfg(const RenderArgs &A, unsigned long long i) {
+    const float *r = MODE == M_ADJSUB ? A.jobs + i * 32 + 30 : A.slots + i * 16 + 10;
+    return __float_as_int(*r);
+}
+
 // A queued sub-path (adj_defer_sub's job layout) as the lane's backward run.
 template <int FIELD>
 __device__ __forceinline__ void job_load(Lane &L, unsigned long long j) {
@@ -346,10 +358,14 @@ __device__ __forceinline__ bool slot_load(Lane &L, unsigned long long s) {
 // main paths.  Every lane runs backwards: items are fetched wave by wave from
 // the queue's own fetch counter, so a wave's lanes step together instead of
 // waiting on the few lanes of a marching wave that reverse.
-template <int FIELD, int MODE>
+// BATCH: items of many point-source patterns; a lane switches to an item's
+// configuration (flushing its parameter sums to the previous one's
+// gradient) as the re-trace does.
+template <int FIELD, int MODE, bool BATCH = false>
 __global__ void __launch_bounds__(kBlock) k_adj_rev(DevScene S, RenderArgs A) {
     static_assert(MODE == M_ADJSUB || MODE == M_ADJMAIN, "backward modes only");
     __shared__ uint32_t cnt[C_N];
+    __shared__ Var vars[BATCH ? kBlock : 1];
     __shared__ float sacc[FIELD == kSources ? (2 * kMaxAdjSources + 1) * kBlock : (FIELD == kBessel ? 3 * kBlock : 1)];
     __shared__ float vcell[FIELD == kSpline ? 64 * kBlock : 1];  // spline VJP cell sums
     if (FIELD == kSpline)
@@ -361,7 +377,9 @@ __global__ void __launch_bounds__(kBlock) k_adj_rev(DevScene S, RenderArgs A) {
     const unsigned long long n_items =
         MODE == M_ADJSUB ? min(A.qcnt[Q_JOBS], A.job_cap) : min(A.qcnt[Q_SLOTS], A.slot_cap);
     unsigned long long *fetch = A.qcnt + (MODE == M_ADJSUB ? Q_WSUB : Q_WMAIN);
-    Var v = var_from(S, A);
+    Var vsingle = var_from(S, A);
+    Var &v = BATCH ? vars[threadIdx.x] : vsingle;
+    int cur_cfg = -1;
     Hot h;
     h.steps = 0;
     h.state = ST_NEED;
@@ -383,7 +401,6 @@ __global__ void __launch_bounds__(kBlock) k_adj_rev(DevScene S, RenderArgs A) {
     a.G.kx = -1;
     Lane L{h, c, &v, S, A, cnt, &a, nullptr, nullptr, nullptr, true};
     const int lane = __lane_id();
-    const FieldP F = v.f;
     while (true) {
         const unsigned long long need = __ballot(h.state == ST_NEED);
         if (need) {
@@ -393,17 +410,28 @@ __global__ void __launch_bounds__(kBlock) k_adj_rev(DevScene S, RenderArgs A) {
             base = __shfl(base, leader);
             if (h.state == ST_NEED) {
                 const unsigned long long idx = base + __popcll(need & ((1ull << lane) - 1ull));
-                if (idx >= n_items)
+                if (idx >= n_items) {
                     h.state = ST_DONE;
-                else if constexpr (MODE == M_ADJSUB)
-                    job_load<FIELD>(L, idx);
-                else if (!slot_load<FIELD>(L, idx))
-                    h.state = ST_NEED;
+                } else {
+                    if constexpr (BATCH) {
+                        const int cfg = item_cfg<MODE>(A, idx);
+                        if (cfg != cur_cfg) {
+                            adj_flush_cfg(a, A, cur_cfg, npar);
+                            cur_cfg = cfg;
+                            v = A.cfgs[cfg];
+                        }
+                    }
+                    if constexpr (MODE == M_ADJSUB)
+                        job_load<FIELD>(L, idx);
+                    else if (!slot_load<FIELD>(L, idx))
+                        h.state = ST_NEED;
+                }
             }
         }
         if (h.state == ST_REND) rev_end<FIELD, MODE>(L);
         if (__ballot(h.state != ST_DONE) == 0ull) break;
         if (__ballot(h.state == ST_RMARCH) == 0ull) continue;
+        const FieldP F = v.f;
         bool r = h.state == ST_RMARCH;
         for (int k = 0; k < A.spc; ++k)
             if (r) r = rev_step<FIELD, FIELD == kSpline>(h, a, S, F, &scell);
@@ -420,8 +448,9 @@ __global__ void __launch_bounds__(kBlock) k_adj_rev(DevScene S, RenderArgs A) {
         for (int k = 0; k < A_N; ++k)
             if (ac[k]) atomicAdd(A.acnt + k, ac[k]);
     }
+    if (BATCH) adj_flush_cfg(a, A, cur_cfg, npar);
     __syncthreads();
-    for (int j = threadIdx.x; j < npar; j += kBlock) {
+    for (int j = threadIdx.x; j < (BATCH ? 0 : npar); j += kBlock) {
         float sum = 0.f;
         for (int t = 0; t < kBlock; ++t) sum += sacc[j * kBlock + t];
         if (sum != 0.f) atomicAdd(A.grad + j, sum);

@@ -1458,6 +1458,8 @@ struct simpler_ctx {
     int64_t adj_chunk = 1 << 22; // deferred adjoint: photons per chunk (adapted to the queues' fill)
     float *d_gx = nullptr;       // spline adjoint: per-XCD gradient copies (kept zero between calls)
     int *d_cut = nullptr;        // spline image adjoint: objective support and coefficient differences
+    double *d_bobj = nullptr;    // batched adjoint: objectives per configuration, then their backup
+    size_t bobj_n = 0;           // configurations d_bobj holds
     size_t gx_floats = 0;        // floats per copy
     float *d_jobs = nullptr, *d_jobrec = nullptr, *d_slots = nullptr, *d_slotrec = nullptr;
     unsigned long long *d_q = nullptr;  // queue counters, then the per-chunk backups
@@ -1889,6 +1891,7 @@ int simpler_ctx_destroy(simpler_ctx *ctx) {
     (void)hipFree(ctx->d_q);
     (void)hipFree(ctx->d_gx);
     (void)hipFree(ctx->d_cut);
+    (void)hipFree(ctx->d_bobj);
     (void)hipFree(ctx->d_pref);
     (void)hipEventDestroy(ctx->handoff);
     (void)hipStreamDestroy(ctx->own);
@@ -2709,18 +2712,21 @@ static int adjoint_gx(simpler_ctx *ctx, int64_t n, RenderArgs &A) {
 }
 
 extern "C++" {
-template <int F>
+template <int F, bool BATCH = false>
 static void launch_adj_inline(simpler_ctx *ctx, dim3 grid, const DevScene &S, const RenderArgs &A) {
-    hipLaunchKernelGGL((k_adjoint<F, M_ADJ>), grid, dim3(kBlock), 0, ctx->cur, S, A);
+    hipLaunchKernelGGL((k_adjoint<F, M_ADJ, BATCH>), grid, dim3(kBlock), 0, ctx->cur, S, A);
 }
 
 // The image adjoint in chunks of photons: re-trace (queueing the backward
 // runs), then the queued sub-paths, then the stored main paths.  A chunk whose
 // queues overflow has its objective and counters restored and is re-traced
 // as a chunk half its size; at the smallest chunk it runs with the backward
-// runs inline.
-template <int F>
-static int adjoint_deferred(simpler_ctx *ctx, dim3 grid, const DevScene &S, RenderArgs A, int64_t n) {
+// runs inline.  BATCH: photons of a pattern batch, indexed over all its
+// configurations; their objectives (n_obj doubles at A.obj) are backed up
+// in obj_backup for each chunk.
+template <int F, bool BATCH = false>
+static int adjoint_deferred(simpler_ctx *ctx, dim3 grid, const DevScene &S, RenderArgs A, int64_t n,
+                            size_t n_obj = 0, double *obj_backup = nullptr) {
     A.jobs = ctx->d_jobs;
     A.jobrec = ctx->d_jobrec;
     A.slots = ctx->d_slots;
@@ -2740,9 +2746,11 @@ static int adjoint_deferred(simpler_ctx *ctx, dim3 grid, const DevScene &S, Rend
                                     hipMemcpyDeviceToDevice, ctx->cur));
         HIP_TRY(ctx, hipMemcpyAsync(ctx->d_q + 24, ctx->d_scratch + 8, 8 * sizeof(unsigned long long),
                                     hipMemcpyDeviceToDevice, ctx->cur));
+        if (BATCH)
+            HIP_TRY(ctx, hipMemcpyAsync(obj_backup, A.obj, n_obj * sizeof(double), hipMemcpyDeviceToDevice, ctx->cur));
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, ctx->d_scratch, 1);
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, ctx->d_q, Q_N);
-        hipLaunchKernelGGL((k_adjoint<F, M_ADJDEF>), grid, dim3(kBlock), 0, ctx->cur, S, A);
+        hipLaunchKernelGGL((k_adjoint<F, M_ADJDEF, BATCH>), grid, dim3(kBlock), 0, ctx->cur, S, A);
         HIP_TRY(ctx, hipGetLastError());
         unsigned long long q[Q_OVER + 1];
         HIP_TRY(ctx, hipMemcpyAsync(q, ctx->d_q, sizeof(q), hipMemcpyDeviceToHost, ctx->cur));
@@ -2752,15 +2760,18 @@ static int adjoint_deferred(simpler_ctx *ctx, dim3 grid, const DevScene &S, Rend
                                         hipMemcpyDeviceToDevice, ctx->cur));
             HIP_TRY(ctx, hipMemcpyAsync(ctx->d_scratch + 8, ctx->d_q + 24, 8 * sizeof(unsigned long long),
                                         hipMemcpyDeviceToDevice, ctx->cur));
+            if (BATCH)
+                HIP_TRY(ctx, hipMemcpyAsync(A.obj, obj_backup, n_obj * sizeof(double), hipMemcpyDeviceToDevice,
+                                            ctx->cur));
             if (m > kAdjChunkMin) {  // re-trace the same photons in a smaller chunk
                 ctx->adj_chunk = std::max<int64_t>(kAdjChunkMin, m / 2);
                 continue;
             }
             hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, ctx->d_scratch, 1);
-            launch_adj_inline<F>(ctx, grid, S, A);
+            launch_adj_inline<F, BATCH>(ctx, grid, S, A);
         } else {
-            hipLaunchKernelGGL((k_adj_rev<F, M_ADJSUB>), rgrid, dim3(kBlock), 0, ctx->cur, S, A);
-            hipLaunchKernelGGL((k_adj_rev<F, M_ADJMAIN>), rgrid, dim3(kBlock), 0, ctx->cur, S, A);
+            hipLaunchKernelGGL((k_adj_rev<F, M_ADJSUB, BATCH>), rgrid, dim3(kBlock), 0, ctx->cur, S, A);
+            hipLaunchKernelGGL((k_adj_rev<F, M_ADJMAIN, BATCH>), rgrid, dim3(kBlock), 0, ctx->cur, S, A);
             double fill = 0.0;
             for (int k = 0; k < 4; ++k) fill = std::max(fill, (double)q[k] / caps[k]);
             if (m == ctx->adj_chunk && fill < 0.4 && ctx->adj_chunk < kAdjChunkMax) ctx->adj_chunk *= 2;
@@ -3195,8 +3206,17 @@ int simpler_batch_adjoint(simpler_batch *b, const float *dF_dimages, float *grad
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     int rc = adjoint_buffers(ctx);
     if (rc) return rc;
-    double *d_obj = nullptr;
-    HIP_TRY(ctx, hipMalloc(&d_obj, (size_t)b->n_cfg * sizeof(double)));
+    // the configurations' objectives and their per-chunk backup, kept by the
+    // context
+    if (ctx->bobj_n < (size_t)b->n_cfg) {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
+        (void)hipFree(ctx->d_bobj);
+        ctx->d_bobj = nullptr;
+        ctx->bobj_n = 0;
+        HIP_TRY(ctx, hipMalloc(&ctx->d_bobj, 2 * (size_t)b->n_cfg * sizeof(double)));
+        ctx->bobj_n = (size_t)b->n_cfg;
+    }
+    double *d_obj = ctx->d_bobj;
     RenderArgs A;
     std::memset(&A, 0, sizeof(A));
     A.X = b->X;
@@ -3226,11 +3246,19 @@ int simpler_batch_adjoint(simpler_batch *b, const float *dF_dimages, float *grad
         hipLaunchKernelGGL(k_fill_f32, dim3((int)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0, ctx->cur,
                            grads, n, 0.0f);
     }
-    if (b->total > 0) {
-        const int blocks = std::max(1, ctx->num_cu * kAdjBlocksPerCU);
-        hipLaunchKernelGGL((k_adjoint<kSources, M_ADJ, true>), dim3(blocks), dim3(kBlock), 0, ctx->cur, b->S, A);
-    }
     rc = SIMPLER_OK;
+    if (b->total > 0) {
+        const dim3 grid(std::max(1, ctx->num_cu * kAdjBlocksPerCU));
+        if (ctx->adj_sched == 1) {
+            rc = adjoint_queues(ctx);
+            if (!rc)
+                rc = adjoint_deferred<kSources, true>(ctx, grid, b->S, A, (int64_t)b->total, (size_t)b->n_cfg,
+                                                       d_obj + b->n_cfg);
+        } else {
+            launch_adj_inline<kSources, true>(ctx, grid, b->S, A);
+        }
+    }
+    if (rc) return rc;
     if (hipGetLastError() != hipSuccess) rc = fail(ctx, SIMPLER_ERR_HIP, "batch adjoint: launch failed");
     if (!rc && (objectives_host || st)) {
         unsigned long long c[8], a[8];
@@ -3249,8 +3277,6 @@ int simpler_batch_adjoint(simpler_batch *b, const float *dF_dimages, float *grad
             st->dropped = a[1 + A_DROPPED];
         }
     }
-    (void)hipStreamSynchronize(ctx->cur);
-    (void)hipFree(d_obj);
     return rc;
 }
 

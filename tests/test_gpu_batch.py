@@ -117,3 +117,46 @@ def test_pattern_batch_matches_single_scene_renders_and_adjoints(gpu_ctx):
         g = g.double().cpu().numpy()
         assert abs(F[i] - Fi) <= 1e-6 * abs(Fi), i
         assert np.allclose(grads[i], g, rtol=2e-3, atol=1e-4 * np.abs(g).max()), i
+
+
+def test_pattern_batch_adjoint_schedules_agree(gpu_ctx):
+    """The batched adjoint (16 patterns x 4096 photons) with its backward runs
+    deferred (the default) and inline, and deferred with queues sized to 70 %
+    of the batch's queued sub-paths: the whole-batch chunk overflows (its
+    objectives and counters are restored) and the photons are traced again
+    in smaller chunks that start inside a configuration.  Same objectives
+    and counts, gradients to the order of float additions."""
+    d, tex, src, amps, phases, budgets, seeds, res = pattern_setup(n_cfg=16, seed=5, photons=(4096,) * 16)
+    base = engine.Scene(gpu_ctx, d, tex, sources=src)
+    batch = engine.Batch.patterns(base, amps, phases, budgets, seeds, res)
+    X, Y, Z = res
+    yy, xx = np.mgrid[0:Y, 0:X]
+    mask = ((xx + 0.5 - X / 2) ** 2 + (yy + 0.5 - Y / 2) ** 2 <= 36).astype(np.float32)
+    G = torch.tensor(np.repeat(mask[None, None], len(budgets), axis=0).repeat(Z, axis=1), device="cuda")
+    out = {}
+    try:
+        gpu_ctx.set_adjoint_schedule(1, 1 << 23)
+        out["deferred"] = batch.adjoint(G)
+        jobs = out["deferred"][2]["sub_paths_reversed"]
+        assert jobs > 1000
+        gpu_ctx.set_adjoint_schedule(0)
+        out["inline"] = batch.adjoint(G)
+        gpu_ctx.set_adjoint_schedule(1, max(64, int(0.7 * jobs)))
+        out["overflow"] = batch.adjoint(G)
+    finally:
+        gpu_ctx.set_adjoint_schedule(1, 1 << 23)
+        batch.close()
+        base.close()
+    gi, Fi, si = out["inline"]
+    gi = gi.double().cpu().numpy()
+    for name in ("deferred", "overflow"):
+        g, F, st = out[name]
+        g = g.double().cpu().numpy()
+        err = np.abs(g - gi).max() / np.abs(gi).max()
+        print(f"patterns {name}: {st}, gradient difference {err:.2e} of the largest entry", flush=True)
+        assert np.allclose(np.asarray(F), np.asarray(Fi), rtol=1e-7, atol=0.0), name
+        for k in ("er_steps", "reverse_steps"):  # kernels compiled apart: rounding may flip a step
+            assert abs(st[k] - si[k]) <= 1e-9 * si[k], (name, k)
+        for k in ("photons", "sub_paths_reversed", "main_paths_reversed", "dropped"):
+            assert st[k] == si[k], (name, k)
+        assert err <= 1e-4, name
