@@ -70,31 +70,6 @@ def test_spline_field_matches_oracle(gpu_ctx):
     assert np.abs(g_g - g_o).max() < 1e-4 * np.abs(g_o).max()
 
 
-def test_spline_field_outside_the_grid_box_matches_oracle(gpu_ctx):
-    """Points up to 1.5 box lengths outside the spline's grid box (in y and z;
-    the scene's gap planes hold n = n_o beyond the x faces): the padded copy's
-    mirrored halo (2 cells below, 3 above), and past it the per-tap mirror
-    wrap, against the oracle's mirror-wrapped evaluation."""
-    d, tex, _, (vals, dims, lo, hi) = scenes.spline_cube(n=24)
-    coeff = O.spline_build3d(vals.reshape(-1), dims)
-    spl = (coeff.astype(np.float32), dims, lo, hi)
-    rng = np.random.default_rng(19)
-    pts = rng.uniform(-0.0049, 0.0049, (3000, 3))
-    pts[:, 1:] = rng.uniform(-0.02, 0.02, (3000, 2))
-    cell = (np.asarray(hi) - np.asarray(lo)) / (np.asarray(dims) - 1)
-    beyond = (pts[:, 1:] < np.asarray(lo)[1:] - 3 * cell[1:]) | (pts[:, 1:] > np.asarray(hi)[1:] + 3 * cell[1:])
-    assert beyond.any(axis=1).sum() > 1000  # past the halo: the per-tap wrap
-    halo = ((pts[:, 1:] < np.asarray(lo)[1:]) | (pts[:, 1:] > np.asarray(hi)[1:])) & ~beyond
-    assert halo.any(axis=1).sum() > 50
-    n_o, g_o = O.OracleScene(d, tex, spline=spl).field(pts)
-    n_g, g_g = engine.Scene(gpu_ctx, d, tex, spline=spl).field(torch.tensor(pts, dtype=torch.float32, device="cuda"))
-    n_g, g_g = n_g.double().cpu().numpy(), g_g.double().cpu().numpy()
-    log(f"spline outside the box: dn {np.abs(n_g - n_o).max():.2e} "
-        f"dg {np.abs(g_g - g_o).max() / np.abs(g_o).max():.2e}")
-    assert np.abs(n_g - n_o).max() < 4e-7
-    assert np.abs(g_g - g_o).max() < 1e-4 * np.abs(g_o).max()
-
-
 @pytest.mark.parametrize("mode", [0, 2])
 def test_transparent_per_ray_exit_matches_fp64(gpu_ctx, mode):
     """configs: sigma_s = 0, per-ray exit p, d, L within 1e-5 relative."""
