@@ -2632,7 +2632,15 @@ int simpler_field_eval(simpler_scene *sc, int n_pts, const float *pts_dev, float
 // ---------------------------------------------------------------- adjoint
 constexpr int kAdjMainCap = 512;  // logged main-path segments per photon
 constexpr int kAdjSubCap = 64;    // logged face reflections per NEE / direct sub-path
-constexpr int kAdjBlocksPerCU = 2;
+constexpr int kAdjBlocksPerCU = 2;  // inline adjoint kernels (one or two waves per SIMD by their registers / LDS)
+// deferred schedule: the re-trace of the spline field fits two workgroups per
+// CU, of the other fields three; their backward kernels fit two (spline,
+// batches: LDS) or four
+__host__ __device__ constexpr int def_trace_blocks(int field) { return field == kSpline ? 2 : 3; }
+__host__ __device__ constexpr int def_rev_blocks(int field, bool batch) {
+    return field == kSpline || batch ? 2 : 4;
+}
+constexpr int kAdjLogBlocksPerCU = 3;  // the segment logs cover the largest of these grids
 
 int simpler_adjoint_param_count(const simpler_scene *sc, int64_t *n) {
     if (!sc || !n) return SIMPLER_ERR_INVALID;
@@ -2646,7 +2654,7 @@ int simpler_adjoint_param_count(const simpler_scene *sc, int64_t *n) {
 // Segment logs (one list per lane of the adjoint grid) and the objective /
 // counter words, allocated on first use and kept by the context.
 static int adjoint_buffers(simpler_ctx *ctx) {
-    const size_t lanes = (size_t)std::max(1, ctx->num_cu * kAdjBlocksPerCU) * kBlock;
+    const size_t lanes = (size_t)std::max(1, ctx->num_cu * kAdjLogBlocksPerCU) * kBlock;
     const size_t bytes = lanes * (size_t)(kAdjMainCap + kAdjSubCap) * kRecFloats * sizeof(float);
     if (ctx->log_bytes < bytes) {
         HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
@@ -2735,7 +2743,8 @@ static int adjoint_deferred(simpler_ctx *ctx, dim3 grid, const DevScene &S, Rend
     A.job_cap = A.jobrec_cap = A.slot_cap = (unsigned long long)ctx->adj_qalloc;
     A.slotrec_cap = 8ull * (unsigned long long)ctx->adj_qalloc;
     const uint64_t off0 = A.offset;
-    const dim3 rgrid(std::max(1, ctx->num_cu * kAdjBlocksPerCU));
+    const dim3 tgrid(std::max(1, ctx->num_cu * def_trace_blocks(F)));
+    const dim3 rgrid(std::max(1, ctx->num_cu * def_rev_blocks(F, BATCH)));
     const double caps[4] = {(double)A.job_cap, (double)A.jobrec_cap, (double)A.slot_cap, (double)A.slotrec_cap};
     int64_t start = 0;
     while (start < n) {
@@ -2750,7 +2759,7 @@ static int adjoint_deferred(simpler_ctx *ctx, dim3 grid, const DevScene &S, Rend
             HIP_TRY(ctx, hipMemcpyAsync(obj_backup, A.obj, n_obj * sizeof(double), hipMemcpyDeviceToDevice, ctx->cur));
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, ctx->d_scratch, 1);
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur, ctx->d_q, Q_N);
-        hipLaunchKernelGGL((k_adjoint<F, M_ADJDEF, BATCH>), grid, dim3(kBlock), 0, ctx->cur, S, A);
+        hipLaunchKernelGGL((k_adjoint<F, M_ADJDEF, BATCH>), tgrid, dim3(kBlock), 0, ctx->cur, S, A);
         HIP_TRY(ctx, hipGetLastError());
         unsigned long long q[Q_OVER + 1];
         HIP_TRY(ctx, hipMemcpyAsync(q, ctx->d_q, sizeof(q), hipMemcpyDeviceToHost, ctx->cur));
