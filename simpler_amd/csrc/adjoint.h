@@ -165,10 +165,11 @@ __device__ __forceinline__ f3 reflect_vec(f3 x, f3 nrm) { return x - (2.f * dot(
 // The start point is p = q - h vh / n(p), solved with n evaluated at
 // q - h vh / n(q) (the error is O(h^3 |grad n|^2), far below FP32).
 // Returns false when the segment start is reached.
-// CELL: the spline taps come from the lane's register copy of the current cell.
-template <int FIELD, bool CELL = false>
+// CELL: the spline taps come from the lane's register copy of the current cell;
+// ACC: the spline parameter derivatives go to the lane's register sums.
+template <int FIELD, bool CELL = false, bool ACC = false>
 __device__ __forceinline__ bool rev_step(Hot &h, AdjState &a, const DevScene &S, const FieldP &F,
-                                         SplineCell *cell = nullptr) {
+                                         SplineCell *cell = nullptr, SplineAcc *acc = nullptr) {
     const float hs = a.rh;
     const float hh = 0.5f * hs;
     a.lp = a.lp + hh * symmul(a.H, a.lv);
@@ -177,7 +178,10 @@ __device__ __forceinline__ bool rev_step(Hot &h, AdjState &a, const DevScene &S,
         a.lp = a.lp + (a.gL * hs) * h.g0;
         a.pa += a.gL * hs;
     }
-    field_vjp<FIELD>(S, F, h.p, h.scaling, h.tau, a.pa, a.pb, a.G);
+    if constexpr (FIELD == kSpline && ACC)
+        spline_vjp_acc(S, h.p, a.pa, a.pb, a.G, *acc);
+    else
+        field_vjp<FIELD>(S, F, h.p, h.scaling, h.tau, a.pa, a.pb, a.G);
     const f3 vh = h.v - hh * h.g0;
     const f3 pt = h.p - (hs / h.n0) * vh;
     float n1;

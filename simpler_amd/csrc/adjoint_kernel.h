@@ -361,6 +361,8 @@ __device__ __forceinline__ bool slot_load(Lane &L, unsigned long long s) {
 // BATCH: items of many point-source patterns; a lane switches to an item's
 // configuration (flushing its parameter sums to the previous one's
 // gradient) as the re-trace does.
+constexpr bool kRevAcc = true;  // spline backward kernels: register tap sums (SplineAcc)
+
 template <int FIELD, int MODE, bool BATCH = false>
 __global__ void __launch_bounds__(kBlock) k_adj_rev(DevScene S, RenderArgs A) {
     static_assert(MODE == M_ADJSUB || MODE == M_ADJMAIN, "backward modes only");
@@ -388,6 +390,12 @@ __global__ void __launch_bounds__(kBlock) k_adj_rev(DevScene S, RenderArgs A) {
     AdjState a;
     SplineCell scell;
     scell.key = -1;
+    constexpr bool ACC = FIELD == kSpline && kRevAcc;
+    SplineAcc racc;
+    racc.kx = kAccEmpty;
+    if constexpr (ACC)
+#pragma unroll
+        for (int t = 0; t < 64; ++t) racc.v[t] = 0.f;
     a.F = 0.0;
     a.rsteps = a.nsubrev = a.nmainrev = a.dropped = 0;
     a.pending = a.trunc_main = a.trunc_sub = false;
@@ -434,9 +442,10 @@ __global__ void __launch_bounds__(kBlock) k_adj_rev(DevScene S, RenderArgs A) {
         const FieldP F = v.f;
         bool r = h.state == ST_RMARCH;
         for (int k = 0; k < A.spc; ++k)
-            if (r) r = rev_step<FIELD, FIELD == kSpline>(h, a, S, F, &scell);
+            if (r) r = rev_step<FIELD, FIELD == kSpline, ACC>(h, a, S, F, &scell, &racc);
         if (h.state == ST_RMARCH && !r) h.state = ST_REND;
     }
+    if constexpr (ACC) spline_acc_commit(S, a.G, racc);
     if (FIELD == kSpline) spline_cell_flush(S, a.G);
     unsigned long long ac[A_N] = {a.rsteps, a.nsubrev, a.nmainrev, a.dropped};
     for (int off = 32; off > 0; off >>= 1) {

@@ -922,6 +922,91 @@ __device__ __forceinline__ void field_vjp<kSpline>(const DevScene &S, const Fiel
 }
 
 
+// A lane's register sums of the 64 tap contributions of one base cell
+// (unclamped base tap (kx, ky, kz); kx = kAccEmpty when empty).  A reverse path
+// stays about ten steps in a cell: summing there and adding the sums to the
+// LDS window once per cell cuts the LDS adds about tenfold.
+constexpr int kAccEmpty = -0x7fffffff - 1;
+struct SplineAcc {
+    float v[64];  // [c][b][q] as field_vjp's taps
+    int kx, ky, kz;
+};
+
+// Adds the register sums to the lane's window (or, where the taps do not fit
+// one window, to the gradient) and empties them.
+__device__ __forceinline__ void spline_acc_commit(const DevScene &S, GradSink &G, SplineAcc &acc) {
+    if (acc.kx == kAccEmpty) return;
+    const int ix = acc.kx, iy = acc.ky, iz = acc.kz;
+    const int NX = S.sN[0], NY = S.sN[1], NZ = S.sN[2];
+    int xi[4], yi[4], zi[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        xi[q] = mirror_idx(ix + q, NX);
+        yi[q] = mirror_idx(iy + q, NY);
+        zi[q] = mirror_idx(iz + q, NZ);
+    }
+    const int kx = min(max(ix, 0), NX - 4), ky = min(max(iy, 0), NY - 4), kz = min(max(iz, 0), NZ - 4);
+    bool fit = G.cell != nullptr && kx >= 0 && ky >= 0 && kz >= 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        fit = fit && (unsigned)(xi[q] - kx) < 4u && (unsigned)(yi[q] - ky) < 4u && (unsigned)(zi[q] - kz) < 4u;
+    if (fit) {
+        if (kx != G.kx || ky != G.ky || kz != G.kz) spline_window_move(S, G, kx, ky, kz);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                float *row = G.cell + (((zi[c] & 3) << 4) | ((yi[b] & 3) << 2)) * G.stride;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    __hip_atomic_fetch_add(row + (xi[q] & 3) * G.stride, acc.v[(c * 4 + b) * 4 + q],
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                float *row = G.grid + ((long)zi[c] * NY + yi[b]) * NX;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) unsafeAtomicAdd(row + xi[q], acc.v[(c * 4 + b) * 4 + q]);
+            }
+    }
+#pragma unroll
+    for (int t = 0; t < 64; ++t) acc.v[t] = 0.f;
+    acc.kx = kAccEmpty;
+}
+
+// field_vjp<kSpline> into the register sums (see SplineAcc).
+__device__ __forceinline__ void spline_vjp_acc(const DevScene &S, f3 p, float a, f3 b, GradSink &G, SplineAcc &acc) {
+    if (p.x > S.egap || p.x < S.sgap) return;
+    const float gx = (p.x - S.sxmin[0]) * S.sxres[0];
+    const float gy = (p.y - S.sxmin[1]) * S.sxres[1];
+    const float gz = (p.z - S.sxmin[2]) * S.sxres[2];
+    const float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
+    float wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
+    bspline_w(gx - fx, wx, dwx);
+    bspline_w(gy - fy, wy, dwy);
+    bspline_w(gz - fz, wz, dwz);
+    const int ix = (int)fx - 1, iy = (int)fy - 1, iz = (int)fz - 1;
+    if (ix != acc.kx || iy != acc.ky || iz != acc.kz) {
+        spline_acc_commit(S, G, acc);
+        acc.kx = ix;
+        acc.ky = iy;
+        acc.kz = iz;
+    }
+    const float bx = b.x * S.sxres[0], by = b.y * S.sxres[1], bz = b.z * S.sxres[2];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int bq = 0; bq < 4; ++bq) {
+            const float Pw = (a * wy[bq] + by * dwy[bq]) * wz[c] + bz * wy[bq] * dwz[c];
+            const float Pd = bx * wy[bq] * wz[c];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc.v[(c * 4 + bq) * 4 + q] += Pw * wx[q] + Pd * dwx[q];
+        }
+}
+
 // ====================================================================
 // Tangent of the field along a parameter direction theta_dot (the forward
 // mode of field_vjp, same parameter layout): nt = sum_j theta_dot_j dn /

@@ -1162,9 +1162,7 @@ k_render(DevScene S, RenderArgs A) {
         bool m = h.state == ST_MARCH;
         uint32_t sl = h.steps_left;
         for (int k = 0; k < A.spc; ++k) {
-            const int na = __popcll(__ballot(m));
-            if (na == 0) break;  // every lane of the chunk has ended its segment
-            wave_steps += na;    // one ER step per marching lane
+            wave_steps += __popcll(__ballot(m));  // one ER step per marching lane
             if constexpr (MODE == M_JVP) {
                 if (m) m = !march_step_jvp<FIELD>(h, tan, sl, S, F, Lcut, A.theta_dot);
             } else {
@@ -2634,11 +2632,11 @@ constexpr int kAdjMainCap = 512;  // logged main-path segments per photon
 constexpr int kAdjSubCap = 64;    // logged face reflections per NEE / direct sub-path
 constexpr int kAdjBlocksPerCU = 2;  // inline adjoint kernels (one or two waves per SIMD by their registers / LDS)
 // deferred schedule: the re-trace of the spline field fits two workgroups per
-// CU, of the other fields three; their backward kernels fit two (spline,
-// batches: LDS) or four
+// CU, of the other fields three; their backward kernels fit one (spline, with
+// the register tap sums), two (batches: LDS) or four
 __host__ __device__ constexpr int def_trace_blocks(int field) { return field == kSpline ? 2 : 3; }
 __host__ __device__ constexpr int def_rev_blocks(int field, bool batch) {
-    return field == kSpline || batch ? 2 : 4;
+    return field == kSpline ? (kRevAcc ? 1 : 2) : (batch ? 2 : 4);
 }
 constexpr int kAdjLogBlocksPerCU = 3;  // the segment logs cover the largest of these grids
 
