@@ -1128,15 +1128,18 @@ k_render(DevScene S, RenderArgs A) {
                 if (idx < A.n) {
                     uint64_t pid = A.offset + idx;
                     if (BATCH) {
-                        int lo = 0, hi = A.n_cfg;  // largest c with prefix[c] <= idx
+                        // index over all configurations (A.offset: the first
+                        // photon of the launch's group of configurations)
+                        const unsigned long long gi = A.offset + idx;
+                        int lo = 0, hi = A.n_cfg;  // largest c with prefix[c] <= gi
                         while (hi - lo > 1) {
                             const int mid = (lo + hi) >> 1;
-                            if (A.prefix[mid] <= idx)
+                            if (A.prefix[mid] <= gi)
                                 lo = mid;
                             else
                                 hi = mid;
                         }
-                        pid = idx - A.prefix[lo];
+                        pid = gi - A.prefix[lo];
                         L.v = const_cast<Var *>(A.cfgs + lo);  // its image pointer is set per launch
                     }
                     L.c.pid_lo = (uint32_t)pid;
@@ -2959,6 +2962,14 @@ struct simpler_batch {
     int field = SIMPLER_FIELD_BESSEL;
     float4 *d_src = nullptr;  // point-source patterns: n_cfg x n_src (x, y, z, amplitude)
     float *d_phase = nullptr;  // and their phases
+    // Bessel batches hold their configurations sorted by order m (stable):
+    // d_img[j] = the caller's index of configuration j (its image), and the
+    // photons of the configurations of order g_mode[k] are [g_start[k],
+    // g_start[k] + g_count[k]) of the batch's photon index
+    int *d_img = nullptr;
+    int n_groups = 0;
+    int g_mode[kMaxMode + 1];
+    uint64_t g_start[kMaxMode + 1], g_count[kMaxMode + 1];
     DevScene S;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -2966,10 +2977,19 @@ struct simpler_batch {
 };
 
 // Each configuration's image pointer in the batch's device parameters
-// (render launches write configuration c into images + c X Y Z).
-__global__ void k_cfg_images(Var *cfgs, int n_cfg, float *images, size_t xyz) {
+// (render launches write configuration c into images + img[c] X Y Z; img
+// null: the identity).
+__global__ void k_cfg_images(Var *cfgs, int n_cfg, const int *img, float *images, size_t xyz) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < n_cfg) cfgs[c].image = images + (size_t)c * xyz;
+    if (c < n_cfg) cfgs[c].image = images + (size_t)(img ? img[c] : c) * xyz;
+}
+
+// A batch's Bessel launch for the configurations of one order: the render
+// kernel specialised on that order where one exists (m <= 2, the shared
+// axis frame, the tables in LDS), else the general one.
+template <int MM>
+static void batch_bessel_launch(dim3 grid, dim3 block, hipStream_t stream, const DevScene &S, const RenderArgs &A) {
+    hipLaunchKernelGGL((k_render<kBessel, M_RENDER, true, true, MM, 2>), grid, block, 0, stream, S, A);
 }
 
 static int batch_launch(simpler_batch *b, float *images, int accumulate, hipStream_t stream) {
@@ -2991,8 +3011,8 @@ static int batch_launch(simpler_batch *b, float *images, int accumulate, hipStre
     A.jtab_base = b->d_jtab;
     A.jtab_len = b->jtab_len;
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, stream, ctx->d_scratch, 16);
-    hipLaunchKernelGGL(k_cfg_images, dim3((b->n_cfg + 255) / 256), dim3(256), 0, stream, b->d_cfgs, b->n_cfg, images,
-                       (size_t)b->X * b->Y * b->Z);
+    hipLaunchKernelGGL(k_cfg_images, dim3((b->n_cfg + 255) / 256), dim3(256), 0, stream, b->d_cfgs, b->n_cfg,
+                       b->d_img, images, (size_t)b->X * b->Y * b->Z);
     if (!accumulate) {
         const size_t npix = (size_t)b->n_cfg * b->X * b->Y * b->Z;
         const int blocks = (int)std::min<size_t>((npix + 255) / 256, 8192);
@@ -3003,9 +3023,22 @@ static int batch_launch(simpler_batch *b, float *images, int accumulate, hipStre
         if (b->field == SIMPLER_FIELD_SOURCES)
             hipLaunchKernelGGL((k_render<kSources, M_RENDER, true>), dim3(blocks), dim3(ctx->block), 0, stream, b->S,
                                A);
-        else if (b->jtab_len <= kJtabLdsBatch && axis_zframe(b->S))  // the geometry's axis frame is shared
-            hipLaunchKernelGGL((k_render<kBessel, M_RENDER, true, true, -1, 2>), dim3(blocks), dim3(ctx->block), 0,
-                               stream, b->S, A);
+        else if (b->jtab_len <= kJtabLdsBatch && axis_zframe(b->S)) {  // the geometry's axis frame is shared
+            // one launch per order, each on its configurations' photons
+            // (the work counter restarts, the event counters add up)
+            for (int k = 0; k < b->n_groups; ++k) {
+                if (k) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, stream, ctx->d_scratch, 1);
+                A.offset = b->g_start[k];
+                A.n = b->g_count[k];
+                const dim3 grid(blocks), block(ctx->block);
+                switch (b->g_mode[k]) {
+                    case 0: batch_bessel_launch<0>(grid, block, stream, b->S, A); break;
+                    case 1: batch_bessel_launch<1>(grid, block, stream, b->S, A); break;
+                    case 2: batch_bessel_launch<2>(grid, block, stream, b->S, A); break;
+                    default: batch_bessel_launch<-1>(grid, block, stream, b->S, A); break;
+                }
+            }
+        }
         else if (b->jtab_len <= kJtabLdsBatch)
             hipLaunchKernelGGL((k_render<kBessel, M_RENDER, true, true>), dim3(blocks), dim3(ctx->block), 0, stream,
                                b->S, A);
@@ -3085,8 +3118,35 @@ int simpler_batch_create(simpler_scene *base, int n_cfg, const simpler_batch_cfg
         mode_off[m] = jtab_all.size();
         jtab_all.insert(jtab_all.end(), mode_tab[m].begin(), mode_tab[m].end());
     }
+    // configurations sorted by order (stable): each order's photons form one
+    // range of the batch's photon index, rendered by a kernel specialised on
+    // that order; a configuration keeps its own photon ids, seed and image
+    std::vector<int> order(n_cfg), img(n_cfg);
+    for (int i = 0; i < n_cfg; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cfg_mode[a] < cfg_mode[c]; });
+    {
+        std::vector<Var> sv(n_cfg);
+        std::vector<int> sm(n_cfg);
+        for (int j = 0; j < n_cfg; ++j) {
+            sv[j] = vars[order[j]];
+            sm[j] = cfg_mode[order[j]];
+            img[j] = order[j];
+            prefix[j + 1] = prefix[j] + (unsigned long long)cfgs[order[j]].n_photons;
+        }
+        vars.swap(sv);
+        cfg_mode.swap(sm);
+    }
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     simpler_batch *b = new simpler_batch();
+    for (int j = 0; j < n_cfg; ++j) {
+        if (b->n_groups == 0 || b->g_mode[b->n_groups - 1] != cfg_mode[j]) {
+            b->g_mode[b->n_groups] = cfg_mode[j];
+            b->g_start[b->n_groups] = prefix[j];
+            b->g_count[b->n_groups] = 0;
+            ++b->n_groups;
+        }
+        b->g_count[b->n_groups - 1] += prefix[j + 1] - prefix[j];
+    }
     b->ctx = ctx;
     b->base = base;
     b->n_cfg = n_cfg;
@@ -3096,7 +3156,8 @@ int simpler_batch_create(simpler_scene *base, int n_cfg, const simpler_batch_cfg
     b->total = prefix[n_cfg];
     if (hipMalloc(&b->d_jtab, jtab_all.size() * sizeof(float)) != hipSuccess ||
         hipMalloc(&b->d_cfgs, vars.size() * sizeof(Var)) != hipSuccess ||
-        hipMalloc(&b->d_prefix, prefix.size() * sizeof(unsigned long long)) != hipSuccess) {
+        hipMalloc(&b->d_prefix, prefix.size() * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&b->d_img, img.size() * sizeof(int)) != hipSuccess) {
         simpler_batch_destroy(b);
         return fail(ctx, SIMPLER_ERR_NOMEM, "batch: device allocation failed");
     }
@@ -3105,7 +3166,8 @@ int simpler_batch_create(simpler_scene *base, int n_cfg, const simpler_batch_cfg
     if (hipMemcpy(b->d_jtab, jtab_all.data(), jtab_all.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_cfgs, vars.data(), vars.size() * sizeof(Var), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_prefix, prefix.data(), prefix.size() * sizeof(unsigned long long), hipMemcpyHostToDevice) !=
-            hipSuccess) {
+            hipSuccess ||
+        hipMemcpy(b->d_img, img.data(), img.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
         simpler_batch_destroy(b);
         return fail(ctx, SIMPLER_ERR_HIP, "batch: upload failed");
     }
@@ -3343,6 +3405,7 @@ int simpler_batch_destroy(simpler_batch *b) {
     (void)hipFree(b->d_jtab);
     (void)hipFree(b->d_src);
     (void)hipFree(b->d_phase);
+    (void)hipFree(b->d_img);
     delete b;
     return SIMPLER_OK;
 }
