@@ -699,41 +699,117 @@ __device__ __forceinline__ void restore_main(Lane &L, float Lval) {
 #include "adjoint.h"
 namespace simpler {
 
+// The next uniform of the lane's current stream, without taking it.
+__device__ __forceinline__ float peek_u(const Lane &L) {
+    const Cold &c = L.c;
+    uint32_t o[4];
+    philox4x32_10(c.pid_lo, c.pid_hi, c.draw >> 2, c.stream, L.v->seed_lo, L.v->seed_hi, o);
+    const uint32_t k = c.draw & 3u;
+    return to_uniform(k == 0 ? o[0] : (k == 1 ? o[1] : (k == 2 ? o[2] : o[3])));
+}
+
+// The largest fresnel_dielectric value over cosI in [i0, i1], cosT in
+// [t0, t1] (all >= 0): each of its two ratios is monotonic in each argument,
+// so its extremes lie at the corners.
+__device__ __forceinline__ float fresnel_upper(float i0, float i1, float t0, float t1, float eta) {
+    float rs = 0.f, rp = 0.f;
+    const float ci[2] = {i0, i1}, ct[2] = {t0, t1};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const float s = (ci[a] - eta * ct[b]) / (ci[a] + eta * ct[b]);
+            const float p = (ct[b] - eta * ci[a]) / (ct[b] + eta * ci[a]);
+            rs = fmaxf(rs, s * s);
+            rp = fmaxf(rp, p * p);
+        }
+    return 0.5f * (rs + rp);
+}
+
 // Image adjoint of a spline field: can the next-event sub-path that starts
 // at the lane's vertex in direction d (unit) reach a pixel with a nonzero
-// objective weight?  Its straight line reaches the sensor face (x = blockL.x,
-// where the host has checked the camera sits, looking along -x) at q0 after a
-// length t.  The field bends the path by at most kappa s^2 / 2 over a length
-// s, with kappa >= |grad n| / n (n >= 1): |dn/dx_a| <= res_a max|c_{i+1} -
-// c_i| along a, the cubic B-spline's derivative being a convex combination
-// of coefficient differences.  If q0 is more than that bound inside the
-// face's edges, the path cannot meet a side face first (the distance to a
-// side face is linear along the line and the bound convex) and leaves
-// through the sensor face within the bound of q0; if every pixel its splat
-// can then touch (two taps per axis) lies outside the objective's support,
-// the sub-path adds nothing to F or to the gradient.  The margin (1e-5 m and
-// a pixel) covers the march's stop past the face (under 1e-9 m), the inside
-// test's eps and rounding.
+// objective weight?  The field bends the path by at most G s^2 / 2 and turns
+// it by at most G s over a length s, with G >= |grad n| >= |grad n| / n
+// (n >= 1): |dn/dx_a| <= res_a max|c_{i+1} - c_i| along a, the cubic
+// B-spline's derivative being a convex combination of coefficient
+// differences.  The path is replayed as a polyline from face to face (the
+// camera sits on the block's -x face, where next-event paths leave, looking
+// along -x): every face it meets must lie more than the bound inside the
+// other faces' planes (a path at distance up to the bound from the line then
+// meets the same face; a mirror reflection, unfolded, keeps the bound).  At a
+// side face the BSDF takes the unnormalised velocity (|v| = n): a robust
+// total reflection mirrors the line and goes on; otherwise the path leaves
+// the block when the face's Fresnel draw -- the stream's next uniform, the
+// first draw of the sub-path's march -- is at least the largest reflectance
+// the bounds allow.  At the sensor face the path adds nothing if every pixel
+// its splat can touch (two taps per axis) lies outside the objective's
+// support.  Any other case is traced.  The margin (1e-5 m and a pixel)
+// covers the march's stop past a face (under 1e-9 m), the inside test's eps
+// and rounding.
 __device__ __forceinline__ bool nee_skip(const Lane &L, f3 d) {
     const int *cut = L.A.nee_cut;
-    if (!cut || !(d.x < -0.05f)) return false;
+    if (!cut) return false;
     const DevScene &S = L.S;
-    const f3 p = L.h.p;
-    const float t = (p.x - S.inL.x) / -d.x;  // the march leaves at the inside test's bound
     const float gx = __int_as_float(cut[4]) * S.sxres[0], gy = __int_as_float(cut[5]) * S.sxres[1],
                 gz = __int_as_float(cut[6]) * S.sxres[2];
-    const float kappa = 1.01f * sqrtf(gx * gx + gy * gy + gz * gz);
-    const float s = 1.05f * t + 1e-5f;
-    const float dev = 0.5f * kappa * s * s + 1e-5f;
-    const f3 q = p + t * d;
-    if (!(q.y > S.blockL.y + dev && q.y < S.blockR.y - dev && q.z > S.blockL.z + dev && q.z < S.blockR.z - dev))
-        return false;
-    const float x = (dot(S.cam_h, q) - S.cam_o.y) / S.view_plane[0] * (float)L.A.X + 0.5f * (float)L.A.X;
-    const float y = (dot(S.cam_v, q) - S.cam_o.z) / S.view_plane[1] * (float)L.A.Y + 0.5f * (float)L.A.Y;
-    const float mx = dev * (float)L.A.X / S.view_plane[0] + 1.f, my = dev * (float)L.A.Y / S.view_plane[1] + 1.f;
-    if (cut[1] < 0) return true;  // no pixel has a nonzero weight
-    return x + mx < (float)(cut[0] - 1) || x - mx >= (float)(cut[1] + 1) || y + my < (float)(cut[2] - 1) ||
-           y - my >= (float)(cut[3] + 1);
+    const float G = 1.01f * sqrtf(gx * gx + gy * gy + gz * gz);
+    const float n0 = L.h.n0;
+    f3 q = L.h.p;
+    float s = 0.f;
+    for (int face = 0; face < 8; ++face) {
+        constexpr float kFar = 3.0e38f;
+        float t = d.x < -1e-6f ? (q.x - S.inL.x) / -d.x : kFar;  // the sensor face
+        int axis = 0;
+        const float ty = d.y > 1e-6f ? (S.inR.y - q.y) / d.y : (d.y < -1e-6f ? (q.y - S.inL.y) / -d.y : kFar);
+        const float tz = d.z > 1e-6f ? (S.inR.z - q.z) / d.z : (d.z < -1e-6f ? (q.z - S.inL.z) / -d.z : kFar);
+        if (ty < t) {
+            t = ty;
+            axis = 1;
+        }
+        if (tz < t) {
+            t = tz;
+            axis = 2;
+        }
+        if (!(t < 1e30f)) return false;
+        s += t;
+        q = q + t * d;
+        const float sl = 1.05f * s + 1e-5f;
+        const float dev = 0.5f * G * sl * sl + 1e-5f;
+        if (!((axis == 0 || (q.x > S.inL.x + dev && q.x < S.inR.x - dev)) &&
+              (axis == 1 || (q.y > S.inL.y + dev && q.y < S.inR.y - dev)) &&
+              (axis == 2 || (q.z > S.inL.z + dev && q.z < S.inR.z - dev))))
+            return false;
+        if (axis == 0) {  // the sensor face: the splat's pixels
+            if (cut[1] < 0) return true;  // no pixel has a nonzero weight
+            const float x = (dot(S.cam_h, q) - S.cam_o.y) / S.view_plane[0] * (float)L.A.X + 0.5f * (float)L.A.X;
+            const float y = (dot(S.cam_v, q) - S.cam_o.z) / S.view_plane[1] * (float)L.A.Y + 0.5f * (float)L.A.Y;
+            const float mx = dev * (float)L.A.X / S.view_plane[0] + 1.f, my = dev * (float)L.A.Y / S.view_plane[1] + 1.f;
+            return x + mx < (float)(cut[0] - 1) || x - mx >= (float)(cut[1] + 1) || y + my < (float)(cut[2] - 1) ||
+                   y - my >= (float)(cut[3] + 1);
+        }
+        // a side face (bsdf_sample with the unnormalised velocity, leaving: eta = 1 / ior)
+        const float cosd = axis == 1 ? fabsf(d.y) : fabsf(d.z);
+        const float dth = G * sl + 1e-5f;  // turn of the path (rad)
+        if (cosd < 0.05f + dth) return false;
+        if (fabsf(1.0f - S.ior) < kEps) return true;  // no interface: it leaves the block
+        const float eta = 1.0f / S.ior;
+        const float dv = G * sl + 1e-6f;  // change of |v| = n along the path
+        const float pp = n0 * sqrtf(fmaxf(0.f, 1.f - cosd * cosd)) / eta;
+        const float dpp = ((n0 + dv) * dth + dv) / eta + 1e-5f;
+        if (pp - dpp > 1.0f) {  // total reflection (no draw): the line goes on mirrored
+            if (axis == 1)
+                d.y = -d.y;
+            else
+                d.z = -d.z;
+            continue;
+        }
+        if (pp + dpp > 0.98f) return false;
+        const float ci = n0 * cosd, dci = dv + (n0 + dv) * dth;
+        const float hi = pp + dpp, lo = fmaxf(0.f, pp - dpp);
+        const float Fup = fresnel_upper(fmaxf(0.f, ci - dci), ci + dci, sqrtf(1.f - hi * hi), sqrtf(1.f - lo * lo), eta);
+        return peek_u(L) >= Fup + 1e-6f;  // it refracts out of the block: no splat
+    }
+    return false;
 }
 
 // photon.cpp:111-116 loop head + scene.cpp:866-893 (NEE start)

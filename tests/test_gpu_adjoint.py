@@ -389,14 +389,15 @@ def test_deferred_schedule_matches_inline(gpu_ctx, kind):
 
 def test_untraced_next_event_paths_leave_the_objective_unchanged(gpu_ctx):
     """The 64^3 spline workload with its 1 mm aperture objective: the image
-    adjoint does not trace the next-event sub-paths whose straight line
-    leaves through the sensor face and whose landing point (widened by the
-    field's bending bound and a pixel) misses the objective's support, so it
-    takes fewer forward steps than the render (measured: 12 % fewer; most
-    next-event directions meet a side face first, and those are traced), and
-    its F still equals sum G * render over the same photons: a sub-path that
-    could have touched a weighted pixel was traced.  With an all-zero
-    objective nothing contributes."""
+    adjoint does not trace the next-event sub-paths that cannot reach the
+    objective's support -- replayed face by face along their straight line,
+    with the field's bending bound: robust total reflections at the side
+    faces, a robust refraction out of a side face by the sub-path's own
+    Fresnel draw, or a landing point on the sensor face whose pixels (widened
+    by the bound and a pixel) miss the support -- so it takes fewer forward
+    steps than the render, and its F still equals sum G * render over the
+    same photons (one splat into the aperture left out would move F by about
+    2.5e-5 of its value).  With an all-zero objective nothing contributes."""
     d, tex, res, spl = scenes.spline_cube_ready(n=64)
     sc = engine.Scene(gpu_ctx, d, tex, spline=spl)
     G = torch.tensor(scenes.aperture_weights(res), device="cuda")
@@ -406,8 +407,8 @@ def test_untraced_next_event_paths_leave_the_objective_unchanged(gpu_ctx):
     Fr = float((img.double() * G.double()).sum())
     log_line = f"F adjoint {F:.9g} render {Fr:.9g}; forward steps adjoint {ast['er_steps']} render {rst['er_steps']}"
     print(log_line, flush=True)
-    assert abs(F - Fr) <= 2e-5 * abs(Fr)
-    assert ast["er_steps"] < 0.95 * rst["er_steps"]
+    assert abs(F - Fr) <= 5e-6 * abs(Fr)
+    assert ast["er_steps"] < 0.9 * rst["er_steps"]
     g0, F0, _ = sc.adjoint_render(torch.zeros_like(G), n, seed=21)
     sc.close()
     assert F0 == 0.0 and float(g0.abs().max()) == 0.0
