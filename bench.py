@@ -199,7 +199,9 @@ def adjoint_bench(args):
     seeded noise, Gaussian-filtered sigma = 4 cells, max |dn| 5e-5), sigma_s
     2/mm, g 0.9, 5 um step, 64x64x32 sensor.  Objective: throughput into a
     1 mm aperture.  Times the forward render and the path-replay adjoint
-    (gradient w.r.t. all 262,144 coefficients) on the same photons.
+    (gradient w.r.t. all 262,144 coefficients) on the same photons; the
+    forward render also with every next-event sub-path traced (the next-event
+    skip off: same image and splat counts, more ER steps).
 
     Accuracy: central differences with common random numbers on K
     coefficients drawn uniformly at random from those whose gradient is at
@@ -232,6 +234,15 @@ def adjoint_bench(args):
     sc.adjoint_render(G, 4096, seed=11)
     (img, fst), t_fwd = timed(lambda: sc.render(N, res, seed=11))
     F_img = float((img.double() * G.double()).sum())
+    ctx.set_next_event_skip(False)
+    (img_full, fst_full), t_full = timed(lambda: sc.render(N, res, seed=11))
+    ctx.set_next_event_skip(True)
+    skip = dict(forward_full_trace_s=t_full, forward_s=t_fwd, speedup=t_full / t_fwd,
+                er_steps=fst["er_steps"], er_steps_full_trace=fst_full["er_steps"],
+                nee_splats=fst["nee_splats"], nee_splats_full_trace=fst_full["nee_splats"],
+                image_max_abs_diff=float((img - img_full).abs().max()), image_max=float(img_full.abs().max()))
+    del img_full
+    eprint(f"next-event skip: {skip}")
     (grad, F, ast), t_adj = timed(lambda: sc.adjoint_render(G, N, seed=11))
     eprint(f"adjoint: {N} paths, forward {t_fwd:.2f}s, adjoint {t_adj:.2f}s, F {F:.6g} (render {F_img:.6g})")
     sc.close()
@@ -297,7 +308,8 @@ def adjoint_bench(args):
                              estimator="mean of the one-step differences within 25 % of their median",
                              seconds=t_fd),
                fd_full_gradient_s=per_fd * n_par, adjoint_speedup_over_fd=per_fd * n_par / t_adj,
-               adjoint_stats=ast, forward_er_steps=fst["er_steps"])
+               adjoint_stats=ast, forward_er_steps=fst["er_steps"], next_event_skip=skip,
+               adjoint_over_full_trace_forward=t_adj / t_full)
     ctx.close()
     return out
 

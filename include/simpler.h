@@ -485,6 +485,23 @@ int simpler_ctx_get_splat(simpler_ctx *ctx, int *strategy);
 int simpler_ctx_set_adjoint_schedule(simpler_ctx *ctx, int schedule, int64_t queue_capacity);
 int simpler_ctx_get_adjoint_schedule(simpler_ctx *ctx, int *schedule, int64_t *queue_capacity);
 
+/* Forward renders (no reference counterpart: the reference traces every
+ * next-event sub-path, scene.cpp:866-893): enable != 0 (the default) leaves
+ * untraced the next-event sub-paths that provably add nothing to the image,
+ * for spline fields whose least coefficient is at least 1, with the camera
+ * on the block's -x face looking along -x and no sensor lens (Bessel and
+ * point-source renders trace every sub-path).  From a bound on |grad n| the sub-path is replayed as a polyline from
+ * face to face: it is skipped when it leaves through a side face (total
+ * reflections on the way are followed; the face's Fresnel draw, taken ahead
+ * from the photon's counter-based stream, is above the largest reflectance
+ * the bound allows) or meets the sensor face outside the image.  The image,
+ * the splat counts and every draw of the photon are those of the full trace;
+ * er_steps and face_events count the work done and drop.  Record renders and
+ * adjoints are not affected (image adjoints of spline fields skip by their
+ * objective's support in any case). */
+int simpler_ctx_set_next_event_skip(simpler_ctx *ctx, int enable);
+int simpler_ctx_get_next_event_skip(simpler_ctx *ctx, int *enable);
+
 /* Autotuner (no reference counterpart: the reference hard-codes its launch,
  * cuda_renderer.cu:763-767): times an n_photons render of the scene at the
  * given resolution for every candidate (block size 128/256, steps per chunk

@@ -105,6 +105,11 @@ struct RenderArgs {
     // largest coefficient difference along each axis (float bits), for
     // nee_skip; null: every next-event sub-path is traced
     const int *nee_cut;
+    // forward renders of spline fields (nee_cut null): nee_fwd != 0 skips
+    // the next-event sub-paths that cannot reach the image, with
+    // nee_g >= |grad n| >= |grad n| / n over the block
+    int nee_fwd;
+    float nee_g;
 };
 
 // Tangent of the photon state along a direction (forward mode): position,
@@ -726,13 +731,14 @@ __device__ __forceinline__ float fresnel_upper(float i0, float i1, float t0, flo
     return 0.5f * (rs + rp);
 }
 
-// Image adjoint of a spline field: can the next-event sub-path that starts
-// at the lane's vertex in direction d (unit) reach a pixel with a nonzero
-// objective weight?  The field bends the path by at most G s^2 / 2 and turns
-// it by at most G s over a length s, with G >= |grad n| >= |grad n| / n
-// (n >= 1): |dn/dx_a| <= res_a max|c_{i+1} - c_i| along a, the cubic
-// B-spline's derivative being a convex combination of coefficient
-// differences.  The path is replayed as a polyline from face to face (the
+// Can the next-event sub-path that starts at the lane's vertex in direction
+// d (unit) reach a pixel with a nonzero objective weight (image adjoint), or
+// the image at all (forward render), in a spline field?  The field bends the
+// path by at most G s^2 / 2 and turns it by at most G s over a length s, with
+// G >= |grad n| >= |grad n| / n (n >= 1): |dn/dx_a| <= res_a
+// max|c_{i+1} - c_i| along a, the cubic B-spline's derivative being a
+// convex combination of coefficient differences (forward renders: the
+// host's nee_g).  The path is replayed as a polyline from face to face (the
 // camera sits on the block's -x face, where next-event paths leave, looking
 // along -x): every face it meets must lie more than the bound inside the
 // other faces' planes (a path at distance up to the bound from the line then
@@ -748,11 +754,25 @@ __device__ __forceinline__ float fresnel_upper(float i0, float i1, float t0, flo
 // and rounding.
 __device__ __forceinline__ bool nee_skip(const Lane &L, f3 d) {
     const int *cut = L.A.nee_cut;
-    if (!cut) return false;
+    if (!cut && !L.A.nee_fwd) return false;
     const DevScene &S = L.S;
-    const float gx = __int_as_float(cut[4]) * S.sxres[0], gy = __int_as_float(cut[5]) * S.sxres[1],
-                gz = __int_as_float(cut[6]) * S.sxres[2];
-    const float G = 1.01f * sqrtf(gx * gx + gy * gy + gz * gz);
+    float G;
+    int bx0, bx1, by0, by1;  // the objective's pixel box
+    if (cut) {
+        const float gx = __int_as_float(cut[4]) * S.sxres[0], gy = __int_as_float(cut[5]) * S.sxres[1],
+                    gz = __int_as_float(cut[6]) * S.sxres[2];
+        G = 1.01f * sqrtf(gx * gx + gy * gy + gz * gz);
+        bx0 = cut[0];
+        bx1 = cut[1];
+        by0 = cut[2];
+        by1 = cut[3];
+    } else {  // a forward render: the whole image
+        G = 1.01f * L.A.nee_g;
+        bx0 = 0;
+        bx1 = L.A.X - 1;
+        by0 = 0;
+        by1 = L.A.Y - 1;
+    }
     const float n0 = L.h.n0;
     f3 q = L.h.p;
     float s = 0.f;
@@ -780,12 +800,12 @@ __device__ __forceinline__ bool nee_skip(const Lane &L, f3 d) {
               (axis == 2 || (q.z > S.inL.z + dev && q.z < S.inR.z - dev))))
             return false;
         if (axis == 0) {  // the sensor face: the splat's pixels
-            if (cut[1] < 0) return true;  // no pixel has a nonzero weight
+            if (bx1 < 0) return true;  // no pixel has a nonzero weight
             const float x = (dot(S.cam_h, q) - S.cam_o.y) / S.view_plane[0] * (float)L.A.X + 0.5f * (float)L.A.X;
             const float y = (dot(S.cam_v, q) - S.cam_o.z) / S.view_plane[1] * (float)L.A.Y + 0.5f * (float)L.A.Y;
             const float mx = dev * (float)L.A.X / S.view_plane[0] + 1.f, my = dev * (float)L.A.Y / S.view_plane[1] + 1.f;
-            return x + mx < (float)(cut[0] - 1) || x - mx >= (float)(cut[1] + 1) || y + my < (float)(cut[2] - 1) ||
-                   y - my >= (float)(cut[3] + 1);
+            return x + mx < (float)(bx0 - 1) || x - mx >= (float)(bx1 + 1) || y + my < (float)(by0 - 1) ||
+                   y - my >= (float)(by1 + 1);
         }
         // a side face (bsdf_sample with the unnormalised velocity, leaving: eta = 1 / ior)
         const float cosd = axis == 1 ? fabsf(d.y) : fabsf(d.z);
@@ -812,7 +832,11 @@ __device__ __forceinline__ bool nee_skip(const Lane &L, f3 d) {
     return false;
 }
 
-// photon.cpp:111-116 loop head + scene.cpp:866-893 (NEE start)
+// photon.cpp:111-116 loop head + scene.cpp:866-893 (NEE start).  Forward
+// renders take nee_skip in the spline kernels only: inlined into the Bessel
+// kernel's event code it slows that kernel's march loop by a fifth per step,
+// more than the 11 % of its steps it removes.
+template <int FIELD>
 __device__ __forceinline__ void vertex(Lane &L) {
     const DevScene &S = L.S;
     Hot &h = L.h;
@@ -841,7 +865,7 @@ __device__ __forceinline__ void vertex(Lane &L) {
         c.acc = 0.f;
         c.large = kLargeDist;
         c.faces = 0;
-        if (L.adj && nee_skip(L, mk3(-z, ca * tmp, sa * tmp))) {  // ends as a dead march: no splat
+        if ((L.adj || (FIELD == kSpline && L.A.nee_fwd)) && nee_skip(L, mk3(-z, ca * tmp, sa * tmp))) {  // no splat
             h.dead = true;
             h.state = ST_SEG_END;
             return;
@@ -987,7 +1011,7 @@ __device__ __forceinline__ void seg_end(Lane &L) {
             ++c.depth;
         }
         if constexpr (adj_image(MODE)) adj_log_main(L, K_DIST, mk3(0.f, 0.f, 0.f));
-        vertex(L);
+        vertex<FIELD>(L);
         return;
     }
     int res;
@@ -1530,6 +1554,7 @@ struct simpler_ctx {
     size_t log_bytes = 0;
     unsigned long long *d_adj = nullptr;  // adjoint objective (double) + counters
     int adj_sched = 1;  // image adjoint: 0 backward runs inline, 1 deferred to k_adj_rev
+    int nee_fwd = 1;    // forward renders: next-event sub-paths that cannot reach the image not traced
     int64_t adj_qcap = 1 << 23;  // deferred queues: sub-paths (and main paths, records) per chunk
     int64_t adj_qalloc = 0;      // the capacity the queues were allocated for
     int64_t adj_chunk = 1 << 22; // deferred adjoint: photons per chunk (adapted to the queues' fill)
@@ -1563,6 +1588,7 @@ struct simpler_scene {
     int sN[3] = {0, 0, 0}, sP[3] = {0, 0, 0};
     float sxmin[3] = {0, 0, 0}, sxmax[3] = {0, 0, 0};
     float spline_min = 0.f;
+    float spline_dmax[3] = {0, 0, 0};  // largest coefficient difference along each axis
     int batches = 0;  // batches built on this scene
 };
 
@@ -2130,6 +2156,18 @@ int simpler_ctx_get_adjoint_schedule(simpler_ctx *ctx, int *schedule, int64_t *q
     return SIMPLER_OK;
 }
 
+int simpler_ctx_set_next_event_skip(simpler_ctx *ctx, int enable) {
+    if (!ctx) return SIMPLER_ERR_INVALID;
+    ctx->nee_fwd = enable ? 1 : 0;
+    return SIMPLER_OK;
+}
+
+int simpler_ctx_get_next_event_skip(simpler_ctx *ctx, int *enable) {
+    if (!ctx || !enable) return SIMPLER_ERR_INVALID;
+    *enable = ctx->nee_fwd;
+    return SIMPLER_OK;
+}
+
 int simpler_ctx_get_tuning(simpler_ctx *ctx, int *block_size, int *steps_per_chunk, int *blocks_per_cu) {
     if (!ctx) return SIMPLER_ERR_INVALID;
     if (block_size) *block_size = ctx->block;
@@ -2326,6 +2364,13 @@ int simpler_scene_set_spline(simpler_scene *sc, const float *coeffs, const int32
     const float ref = (float)(mean / (double)n);
     std::vector<float> rel(n);
     for (size_t i = 0; i < n; ++i) rel[i] = (float)((double)coeffs[i] - (double)ref);
+    float dmax[3] = {0.f, 0.f, 0.f};
+    const size_t row = (size_t)dims[0], slab = row * dims[1];
+    for (size_t i = 0; i < n; ++i) {
+        if ((i % row) + 1 < row) dmax[0] = std::max(dmax[0], std::fabs(rel[i + 1] - rel[i]));
+        if ((i / row) % (size_t)dims[1] + 1 < (size_t)dims[1]) dmax[1] = std::max(dmax[1], std::fabs(rel[i + row] - rel[i]));
+        if (i / slab + 1 < (size_t)dims[2]) dmax[2] = std::max(dmax[2], std::fabs(rel[i + slab] - rel[i]));
+    }
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->cur));
     (void)hipFree(sc->d_spline);
@@ -2336,6 +2381,7 @@ int simpler_scene_set_spline(simpler_scene *sc, const float *coeffs, const int32
     HIP_TRY(ctx, hipMemcpy(sc->d_spline, rel.data(), n * sizeof(float), hipMemcpyHostToDevice));
     sc->spline_ref = ref;
     sc->spline_min = mn;
+    for (int i = 0; i < 3; ++i) sc->spline_dmax[i] = dmax[i];
     for (int i = 0; i < 3; ++i) {
         sc->sN[i] = dims[i];
         sc->sxmin[i] = xmin[i];
@@ -2353,7 +2399,9 @@ static int check_grid(simpler_ctx *ctx, const int32_t dims[3], const float *xmin
 }
 
 static int prefilter_buffer(simpler_ctx *ctx) {
-    if (!ctx->d_pref) HIP_TRY(ctx, hipMalloc(&ctx->d_pref, 2 * sizeof(unsigned)));
+    // [0] non-finite flag, [1] least coefficient (prefilter), [2..4] largest
+    // coefficient differences (k_spline_dmax)
+    if (!ctx->d_pref) HIP_TRY(ctx, hipMalloc(&ctx->d_pref, 8 * sizeof(unsigned)));
     return SIMPLER_OK;
 }
 
@@ -2388,7 +2436,7 @@ int simpler_scene_set_spline_samples(simpler_scene *sc, const float *samples, in
     // the first sample: any value inside the data's range keeps FP32
     // resolution on the variation
     float shift = 0.f;
-    unsigned w[2] = {0u, 0u};
+    unsigned w[5] = {0u, 0u, 0u, 0u, 0u};
     hipError_t e = hipMemcpyAsync(d, samples, n * sizeof(float),
                                   samples_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->cur);
     if (e == hipSuccess) e = hipMemcpyAsync(&shift, d, sizeof(float), hipMemcpyDeviceToHost, ctx->cur);
@@ -2399,6 +2447,13 @@ int simpler_scene_set_spline_samples(simpler_scene *sc, const float *samples, in
     }
     const int N[3] = {dims[0], dims[1], dims[2]};
     if (e == hipSuccess) e = launch_prefilter3d(d, N, shift, ctx->d_pref, ctx->cur);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->cur,
+                           reinterpret_cast<unsigned long long *>(ctx->d_pref + 2), 2);
+        hipLaunchKernelGGL(k_spline_dmax, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0,
+                           ctx->cur, d, N[0], N[1], N[2], ctx->d_pref + 2);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipMemcpyAsync(w, ctx->d_pref, sizeof(w), hipMemcpyDeviceToHost, ctx->cur);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->cur);
     if (e != hipSuccess) {
@@ -2412,6 +2467,7 @@ int simpler_scene_set_spline_samples(simpler_scene *sc, const float *samples, in
     sc->d_spline = d;
     sc->spline_ref = shift;
     sc->spline_min = key_to_float(w[1]) + shift;
+    for (int i = 0; i < 3; ++i) std::memcpy(&sc->spline_dmax[i], &w[2 + i], sizeof(float));
     for (int i = 0; i < 3; ++i) {
         sc->sN[i] = dims[i];
         sc->sxmin[i] = xmin[i];
@@ -2442,6 +2498,31 @@ int simpler_scene_get_spline(const simpler_scene *sc, float *coeffs_host) {
     (void)hipFree(tmp);
     if (e != hipSuccess) return fail(ctx, SIMPLER_ERR_HIP, std::string("spline readback: ") + hipGetErrorString(e));
     return SIMPLER_OK;
+}
+
+// nee_skip's geometry: the camera on the block's -x face looking along -x
+// (where the next-event hemisphere points) and no sensor lens (the splat is
+// then where the path meets the face).
+static bool nee_camera_on_face(const simpler_scene_desc &d) {
+    return d.view_dir[0] < 0.f && d.view_dir[1] == 0.f && d.view_dir[2] == 0.f && d.view_origin[0] == d.block_l[0] &&
+           !d.sensor_lens_active;
+}
+
+// Forward renders of spline fields whose least coefficient (hence index) is
+// at least 1: nee_skip with the whole image as the objective and the bound
+// |grad n| <= G = |(res_a max|c_{i+1} - c_i| along a)_a|, which also bounds
+// |grad n| / n.
+static void nee_forward(const simpler_ctx *ctx, const simpler_scene *sc, const DevScene &S, RenderArgs &A) {
+    const simpler_scene_desc &d = sc->desc;
+    if (!ctx->nee_fwd || !S.use_nee || !nee_camera_on_face(d)) return;
+    if (d.field_kind != SIMPLER_FIELD_SPLINE || !(sc->spline_min >= 1.f)) return;
+    float g2 = 0.f;
+    for (int i = 0; i < 3; ++i) g2 += (sc->spline_dmax[i] * S.sxres[i]) * (sc->spline_dmax[i] * S.sxres[i]);
+    const float G = std::sqrt(g2);
+    if (std::isfinite(G)) {
+        A.nee_fwd = 1;
+        A.nee_g = G;
+    }
 }
 
 // words: the launch's work queue ([0]) and counters ([8..15]); the
@@ -2488,10 +2569,12 @@ static int render_impl(simpler_scene *sc, int64_t n_photons, int64_t n_total, ui
         hipLaunchKernelGGL(k_fill_f32, dim3(blocks), dim3(256), 0, ctx->cur, image, npix, 0.0f);
     }
     if (n_photons > 0) {
-        if (record)
+        if (record) {
             launch_render<M_RECORD>(ctx, sc->desc.field_kind, S, A);
-        else
+        } else {
+            nee_forward(ctx, sc, S, A);
             launch_render<M_RENDER>(ctx, sc->desc.field_kind, S, A);
+        }
     }
     HIP_TRY(ctx, hipGetLastError());
     return SIMPLER_OK;
@@ -2939,11 +3022,9 @@ static int adjoint_impl(simpler_scene *sc, int mode, int64_t n, int64_t n_total,
         rc = adjoint_gx(ctx, npar, A);
         if (rc) return rc;
         // next-event sub-paths that cannot reach the objective are not
-        // traced (nee_skip): spline fields, the camera on the block's -x face
-        // looking along -x (where the next-event hemisphere points)
-        const bool cam_on_face = d.view_dir[0] < 0.f && d.view_dir[1] == 0.f && d.view_dir[2] == 0.f &&
-                                 d.view_origin[0] == d.block_l[0];
-        if (mode == M_ADJ && cam_on_face) {
+        // traced (nee_skip): spline fields whose least coefficient (hence
+        // index) is at least 1, the camera on the block's -x face
+        if (mode == M_ADJ && nee_camera_on_face(d) && sc->spline_min >= 1.f) {
             if (!ctx->d_cut) HIP_TRY(ctx, hipMalloc(&ctx->d_cut, 8 * sizeof(int)));
             hipLaunchKernelGGL(k_cut_init, dim3(1), dim3(64), 0, ctx->cur, ctx->d_cut);
             const size_t ng = (size_t)X * Y * Z, nc = (size_t)sc->sN[0] * sc->sN[1] * sc->sN[2];

@@ -68,6 +68,8 @@ def load_library():
     L.simpler_ctx_get_splat.argtypes = [P, C.POINTER(C.c_int)]
     L.simpler_ctx_set_adjoint_schedule.argtypes = [P, C.c_int, C.c_int64]
     L.simpler_ctx_get_adjoint_schedule.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+    L.simpler_ctx_set_next_event_skip.argtypes = [P, C.c_int]
+    L.simpler_ctx_get_next_event_skip.argtypes = [P, C.POINTER(C.c_int)]
     L.simpler_scene_create.argtypes = [P, C.POINTER(SceneDesc), fptr, C.c_int, C.c_int, PP]
     L.simpler_scene_destroy.argtypes = [P]
     L.simpler_scene_update.argtypes = [P, C.POINTER(SceneDesc)]
@@ -105,7 +107,8 @@ def load_library():
                                        C.POINTER(C.c_double), C.POINTER(AdjointStats)]
     for name in ("simpler_ctx_create", "simpler_ctx_destroy", "simpler_ctx_set_stream", "simpler_ctx_synchronize",
                  "simpler_ctx_set_tuning", "simpler_ctx_get_tuning", "simpler_ctx_set_splat", "simpler_ctx_get_splat",
-                 "simpler_ctx_set_adjoint_schedule", "simpler_ctx_get_adjoint_schedule", "simpler_scene_create", "simpler_scene_destroy",
+                 "simpler_ctx_set_adjoint_schedule", "simpler_ctx_get_adjoint_schedule",
+                 "simpler_ctx_set_next_event_skip", "simpler_ctx_get_next_event_skip", "simpler_scene_create", "simpler_scene_destroy",
                  "simpler_scene_update", "simpler_scene_get_desc", "simpler_scene_set_sources",
                  "simpler_scene_set_spline", "simpler_render", "simpler_render_host", "simpler_trace_rays", "simpler_jvp_rays", "simpler_nee_connect",
                  "simpler_field_eval", "simpler_read_stats", "simpler_render_host_noalloc", "simpler_batch_create", "simpler_batch_render",
@@ -204,13 +207,25 @@ class Context:
     def set_adjoint_schedule(self, schedule, queue_capacity=0):
         """Image adjoint: 0 runs the backward runs inline in the re-trace, 1
         defers them to full-wave backward kernels; queue_capacity sizes the
-        deferred queues per chunk of 2^20 photons (0 keeps it)."""
+        deferred queues of one chunk of photons (0 keeps it)."""
         self.check(self.L.simpler_ctx_set_adjoint_schedule(self.h, int(schedule), int(queue_capacity)))
 
     def get_adjoint_schedule(self):
         s, c = C.c_int(), C.c_int64()
         self.check(self.L.simpler_ctx_get_adjoint_schedule(self.h, C.byref(s), C.byref(c)))
         return dict(schedule=s.value, queue_capacity=c.value)
+
+    def set_next_event_skip(self, enable):
+        """Forward renders of spline fields: leave untraced the next-event
+        sub-paths that provably add nothing to the image (default on; the
+        image and splat counts are those of the full trace, er_steps and
+        face_events drop)."""
+        self.check(self.L.simpler_ctx_set_next_event_skip(self.h, 1 if enable else 0))
+
+    def get_next_event_skip(self):
+        e = C.c_int()
+        self.check(self.L.simpler_ctx_get_next_event_skip(self.h, C.byref(e)))
+        return bool(e.value)
 
     def read_stats(self):
         self._bind()

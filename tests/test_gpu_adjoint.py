@@ -15,6 +15,7 @@ from conftest import gpu_threads
 from oracle import oracle as O
 from simpler_amd import engine, scenes
 from simpler_amd._abi import FLAG_GENERAL_AXIS
+from test_gpu_parity import full_trace
 
 pytestmark = pytest.mark.gpu
 
@@ -395,14 +396,15 @@ def test_untraced_next_event_paths_leave_the_objective_unchanged(gpu_ctx):
     faces, a robust refraction out of a side face by the sub-path's own
     Fresnel draw, or a landing point on the sensor face whose pixels (widened
     by the bound and a pixel) miss the support -- so it takes fewer forward
-    steps than the render, and its F still equals sum G * render over the
-    same photons (one splat into the aperture left out would move F by about
+    steps than a render that traces them all, and its F still equals
+    sum G * render over the same photons (one splat into the aperture left out would move F by about
     2.5e-5 of its value).  With an all-zero objective nothing contributes."""
     d, tex, res, spl = scenes.spline_cube_ready(n=64)
     sc = engine.Scene(gpu_ctx, d, tex, spline=spl)
     G = torch.tensor(scenes.aperture_weights(res), device="cuda")
     n = 1 << 16
-    img, rst = sc.render(n, res, seed=21)
+    with full_trace(gpu_ctx):
+        img, rst = sc.render(n, res, seed=21)
     _, F, ast = sc.adjoint_render(G, n, seed=21)
     Fr = float((img.double() * G.double()).sum())
     log_line = f"F adjoint {F:.9g} render {Fr:.9g}; forward steps adjoint {ast['er_steps']} render {rst['er_steps']}"

@@ -20,7 +20,17 @@ Renderer options (reference scene.cpp:804-805,910-915; photon.cpp:62-67,
 an image vs the FP32 oracle.  The 64^3 differentiable spline field:
 forward image vs the FP32 oracle, and the field at points up to 1.5 grid
 lengths outside the grid (the mirrored halo and the wrapping path).
+
+Next-event skip: forward renders of spline fields leave untraced the
+next-event sub-paths that provably leave the block through a side face or
+meet the sensor face outside the image; against the full trace of the same
+photons the splat counts are equal and the image equal up to the order of
+float additions.
+Comparisons of the work counters (er_steps, face_events) with the oracle
+run the full trace.
 """
+import contextlib
+
 import numpy as np
 import pytest
 import torch
@@ -29,19 +39,22 @@ from conftest import gpu_threads
 from oracle import oracle as O
 from simpler_amd import engine, scenes
 from simpler_amd._abi import FLAG_NO_DEAD_CUT, FLAG_USE_FRESNEL
-from test_gpu_parity import _batches, _bin_check, log
+from test_gpu_parity import _batches, _bin_check, full_trace, log
 
 pytestmark = pytest.mark.gpu
 
 
-def _parity(gpu_ctx, d, tex, res, B, n, seed, what, spline=None, count_tol=0.0, counts=None):
+def _parity(gpu_ctx, d, tex, res, B, n, seed, what, spline=None, count_tol=0.0, counts=None, traced=False):
+    """GPU vs oracle on the same photons, B batches of n.  traced: the GPU
+    renders trace every next-event sub-path (full_trace)."""
     gs = engine.Scene(gpu_ctx, d, tex, spline=spline)
     osc = O.OracleScene(d, tex, spline=spline)
     N = B * n
     g_st, o_st = [], []
 
     def rg(b):
-        img, st = gs.render(n, res, seed=seed, offset=b * n, n_total=N)
+        with full_trace(gpu_ctx) if traced else contextlib.nullcontext():
+            img, st = gs.render(n, res, seed=seed, offset=b * n, n_total=N)
         g_st.append(st)
         return img.double().cpu().numpy()
 
@@ -105,11 +118,12 @@ def test_full_size_dead_path_cutoff_leaves_gpu_image_unchanged(gpu_ctx, mode):
     _, ocut = osc.render(n, res, seed=seed, offset=off, n_total=n, precision=32, threads=gpu_threads(), cutoff=True)
     assert ofull["er_steps"] < 2e7  # the block's full-work render is bounded
     sc = engine.Scene(gpu_ctx, d, tex)
-    cut, scut = sc.render(n, res, seed=seed, offset=off, n_total=n)
     d2 = d.copy()
     d2.flags |= FLAG_NO_DEAD_CUT
     sc2 = engine.Scene(gpu_ctx, d2, tex)
-    full, sfull = sc2.render(n, res, seed=seed, offset=off, n_total=n)
+    with full_trace(gpu_ctx):
+        cut, scut = sc.render(n, res, seed=seed, offset=off, n_total=n)
+        full, sfull = sc2.render(n, res, seed=seed, offset=off, n_total=n)
     cut, full = cut.double().cpu().numpy(), full.double().cpu().numpy()
     log(f"cutoff m={mode}: steps gpu {scut['er_steps']} / {sfull['er_steps']} (cut / full), oracle "
         f"{ocut['er_steps']} / {ofull['er_steps']}; image max diff {np.abs(cut - full).max():.3g} of {full.max():.3g}")
@@ -155,10 +169,49 @@ def test_high_index_cube_rare_branches_within_3_sigma_of_oracle(gpu_ctx):
     d, tex, res = scenes.bessel_cube_small()
     d.set(ior=2.4, sigma_t=1000.0)
     tot = {}
-    _parity(gpu_ctx, d, tex, res, 8, 256, 43, "ior 2.4", count_tol=1e-3, counts=tot)
+    _parity(gpu_ctx, d, tex, res, 8, 256, 43, "ior 2.4", count_tol=1e-3, counts=tot, traced=True)
     fg, fo = tot["face_events"]
     assert fo > 50 * 2048
     assert abs(fg - fo) <= 2e-3 * fo
+
+
+@pytest.mark.parametrize("case", ["spline64", "spline64_ior2.4", "bessel_m2"])
+def test_next_event_skip_leaves_the_image_unchanged(gpu_ctx, case):
+    """The same photons with and without the next-event skip: every count
+    but the work counters equal, the image equal up to the order of float
+    additions (the skipped sub-paths splat nothing), fewer ER steps.  At
+    index 2.4 most next-event sub-paths reflect totally at the side faces,
+    which the skip's replay follows.  Bessel renders trace every sub-path:
+    the setting changes nothing there."""
+    spl = None
+    if case.startswith("spline64"):
+        d, tex, res, spl = scenes.spline_cube_ready(n=64)
+        if case.endswith("ior2.4"):
+            d.set(ior=2.4)
+        n = 1 << 18
+    else:
+        d, tex, res = scenes.bessel_cube_full(2)
+        n = 1 << 18
+    sc = engine.Scene(gpu_ctx, d, tex, spline=spl)
+    assert gpu_ctx.get_next_event_skip()
+    a, son = sc.render(n, res, seed=17)
+    with full_trace(gpu_ctx):
+        b, soff = sc.render(n, res, seed=17)
+    sc.close()
+    a, b = a.double().cpu().numpy(), b.double().cpu().numpy()
+    log(f"next-event skip {case}: steps {son['er_steps']} / {soff['er_steps']} "
+        f"({1 - son['er_steps'] / soff['er_steps']:.1%} fewer), face events {son['face_events']} / "
+        f"{soff['face_events']}, nee splats {son['nee_splats']} / {soff['nee_splats']}, "
+        f"image max diff {np.abs(a - b).max():.3g} of {b.max():.3g}")
+    for k in ("photons", "scatters", "direct_splats", "nee_splats", "aborted"):
+        assert son[k] == soff[k], k
+    if spl is None:
+        assert son == soff
+    else:
+        assert son["er_steps"] < soff["er_steps"] and son["face_events"] < soff["face_events"]
+    assert b.sum() > 0
+    assert abs(a.sum() - b.sum()) <= 1e-6 * b.sum()
+    assert np.abs(a - b).max() <= 1e-5 * b.max()
 
 
 def test_spline_64cube_image_within_3_sigma_of_oracle(gpu_ctx):

@@ -10,6 +10,7 @@
 * properties at full size: chunked renders add up, renders are
            deterministic, the dead-path cutoff leaves the image unchanged.
 """
+import contextlib
 import math
 
 import numpy as np
@@ -26,6 +27,18 @@ pytestmark = pytest.mark.gpu
 
 def log(*a):
     print(*a, flush=True)
+
+
+@contextlib.contextmanager
+def full_trace(ctx):
+    """Forward renders that trace every next-event sub-path, as the oracle
+    does: for comparing the work counters (er_steps, face_events), which
+    count the sub-paths the next-event skip leaves untraced."""
+    ctx.set_next_event_skip(False)
+    try:
+        yield
+    finally:
+        ctx.set_next_event_skip(True)
 
 
 def _pts(n=2000, seed=0, half=0.005):
@@ -304,10 +317,11 @@ def test_dead_path_cutoff_leaves_gpu_image_unchanged(gpu_ctx):
     # the oracle runs the same photons first: a bounded full-work render
     _, ost = O.OracleScene(d, tex).render(n, res, seed=seed, precision=32, threads=gpu_threads(), cutoff=False)
     assert ost["er_steps"] < 5e8
-    cut, _ = engine.Scene(gpu_ctx, d, tex).render(n, res, seed=seed)
     d2 = d.copy()
     d2.flags |= FLAG_NO_DEAD_CUT
-    full, st = engine.Scene(gpu_ctx, d2, tex).render(n, res, seed=seed)
+    with full_trace(gpu_ctx):
+        cut, _ = engine.Scene(gpu_ctx, d, tex).render(n, res, seed=seed)
+        full, st = engine.Scene(gpu_ctx, d2, tex).render(n, res, seed=seed)
     cut, full = cut.double().cpu().numpy(), full.double().cpu().numpy()
     assert np.abs(cut - full).max() <= 1e-6 * full.max()
     assert abs(st["er_steps"] - ost["er_steps"]) < 1e-3 * ost["er_steps"]
