@@ -100,9 +100,10 @@ struct RenderArgs {
     // floats apart), summed into grad by k_gx_reduce
     float *gx;
     size_t gx_stride;
-    // spline image adjoint: the objective's support (pixel box over all
-    // planes: min x, max x, min y, max y; max x < 0 when it is empty) and the
-    // largest coefficient difference along each axis (float bits), for
+    // image adjoint: the objective's support (pixel box over all planes:
+    // min x, max x, min y, max y; max x < 0 when it is empty), the largest
+    // spline coefficient difference along each axis (float bits) and a
+    // bound on |grad n| from the host (float bits; point-source fields), for
     // nee_skip; null: every next-event sub-path is traced
     const int *nee_cut;
     // forward renders of spline fields (nee_cut null): nee_fwd != 0 skips
@@ -732,13 +733,14 @@ __device__ __forceinline__ float fresnel_upper(float i0, float i1, float t0, flo
 }
 
 // Can the next-event sub-path that starts at the lane's vertex in direction
-// d (unit) reach a pixel with a nonzero objective weight (image adjoint), or
-// the image at all (forward render), in a spline field?  The field bends the
-// path by at most G s^2 / 2 and turns it by at most G s over a length s, with
-// G >= |grad n| >= |grad n| / n (n >= 1): |dn/dx_a| <= res_a
-// max|c_{i+1} - c_i| along a, the cubic B-spline's derivative being a
-// convex combination of coefficient differences (forward renders: the
-// host's nee_g).  The path is replayed as a polyline from face to face (the
+// d (unit) reach a pixel with a nonzero objective weight (image adjoint of a
+// spline or point-source field), or the image at all (forward render of a
+// spline field)?  The field bends the path by at most G s^2 / 2 and turns it
+// by at most G s over a length s, with G >= |grad n| >= |grad n| / n: for
+// splines (n >= 1) |dn/dx_a| <= res_a max|c_{i+1} - c_i| along a, the cubic
+// B-spline's derivative being a convex combination of coefficient
+// differences; point-source fields and forward renders take the host's
+// bound (cut[7], nee_g).  The path is replayed as a polyline from face to face (the
 // camera sits on the block's -x face, where next-event paths leave, looking
 // along -x): every face it meets must lie more than the bound inside the
 // other faces' planes (a path at distance up to the bound from the line then
@@ -761,7 +763,7 @@ __device__ __forceinline__ bool nee_skip(const Lane &L, f3 d) {
     if (cut) {
         const float gx = __int_as_float(cut[4]) * S.sxres[0], gy = __int_as_float(cut[5]) * S.sxres[1],
                     gz = __int_as_float(cut[6]) * S.sxres[2];
-        G = 1.01f * sqrtf(gx * gx + gy * gy + gz * gz);
+        G = 1.01f * (sqrtf(gx * gx + gy * gy + gz * gz) + __int_as_float(cut[7]));
         bx0 = cut[0];
         bx1 = cut[1];
         by0 = cut[2];
@@ -1340,9 +1342,10 @@ __global__ void k_spline_dmax(const float *c, int N0, int N1, int N2, unsigned *
     }
 }
 
-// Initial values of the k_gbox / k_spline_dmax accumulators.
-__global__ void k_cut_init(int *cut) {
-    const int init[8] = {0x7fffffff, -1, 0x7fffffff, -1, 0, 0, 0, 0};
+// Initial values of the k_gbox / k_spline_dmax accumulators, and a bound g
+// on |grad n| that the host knows (point-source fields; 0 otherwise).
+__global__ void k_cut_init(int *cut, float g) {
+    const int init[8] = {0x7fffffff, -1, 0x7fffffff, -1, 0, 0, 0, __float_as_int(g)};
     if (threadIdx.x < 8) cut[threadIdx.x] = init[threadIdx.x];
 }
 
@@ -2500,6 +2503,37 @@ int simpler_scene_get_spline(const simpler_scene *sc, float *coeffs_host) {
     return SIMPLER_OK;
 }
 
+// A bound on |grad n| and |grad n| / n over the block for a point-source
+// field with the amplitudes amp (n_src values; null: the scene's).  With
+// n = n_o + sum_i a_i sum_c w_c cos(c k r_i + phi_i) / r_i,
+// |grad n| <= sum_i |a_i| sum_c |w_c| (1 / d_i^2 + c k / d_i), d_i the
+// source's distance from the block; divided by the least index
+// n_o - sum_i |a_i| sum_c |w_c| / d_i when that is below 1.  Negative when a
+// source touches the block or the least index is not positive.
+static double sources_grad_bound(const simpler_scene *sc, const float *amp) {
+    const simpler_scene_desc &d = sc->desc;
+    const double k = 2.0 * 3.14159265358979323846 * (double)d.f_u / (double)d.speed_u;
+    double g = 0.0, dn = 0.0;
+    for (int i = 0; i < sc->n_src; ++i) {
+        double dist2 = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            const double x = sc->h_src[4 * i + a];
+            const double e = x < d.block_l[a] ? d.block_l[a] - x : (x > d.block_r[a] ? x - d.block_r[a] : 0.0);
+            dist2 += e * e;
+        }
+        if (!(dist2 > 0.0)) return -1.0;
+        const double di = std::sqrt(dist2), ai = std::fabs((double)(amp ? amp[i] : sc->h_src[4 * i + 3]));
+        for (int c = 0; c < sc->n_coeff; ++c) {
+            const double w = std::fabs((double)sc->coeff_w[c]) * ai;
+            g += w * (1.0 / dist2 + (double)(c + 1) * std::fabs(k) / di);
+            dn += w / di;
+        }
+    }
+    const double nmin = (double)d.n_o - dn;
+    if (!(nmin > 0.0) || !std::isfinite(g)) return -1.0;
+    return g / std::min(1.0, nmin);
+}
+
 // nee_skip's geometry: the camera on the block's -x face looking along -x
 // (where the next-event hemisphere points) and no sensor lens (the splat is
 // then where the path meets the face).
@@ -2952,6 +2986,19 @@ static int adjoint_deferred(simpler_ctx *ctx, dim3 grid, const DevScene &S, Rend
 }
 }  // extern "C++"
 
+// nee_skip's record for an image adjoint (ctx->d_cut, allocated on first
+// use): the support of the objective weights G (Z planes of X x Y), zeroed
+// spline differences and the host's bound g on |grad n|.
+static int nee_cut_begin(simpler_ctx *ctx, const float *G, int X, int Y, int Z, float g) {
+    if (!ctx->d_cut) HIP_TRY(ctx, hipMalloc(&ctx->d_cut, 8 * sizeof(int)));
+    hipLaunchKernelGGL(k_cut_init, dim3(1), dim3(64), 0, ctx->cur, ctx->d_cut, g);
+    const size_t ng = (size_t)X * Y * Z;
+    hipLaunchKernelGGL(k_gbox, dim3((unsigned)std::min<size_t>((ng + 255) / 256, 4096)), dim3(256), 0, ctx->cur, G,
+                       X, Y, Z, ctx->d_cut);
+    HIP_TRY(ctx, hipGetLastError());
+    return SIMPLER_OK;
+}
+
 static int adjoint_impl(simpler_scene *sc, int mode, int64_t n, int64_t n_total, uint64_t seed, uint64_t offset,
                         const float *G, int X, int Y, int Z, const float *up_p, const float *up_d, const float *up_L,
                         float *grad, int accumulate, double *obj_host, simpler_adjoint_stats *st) {
@@ -3025,15 +3072,22 @@ static int adjoint_impl(simpler_scene *sc, int mode, int64_t n, int64_t n_total,
         // traced (nee_skip): spline fields whose least coefficient (hence
         // index) is at least 1, the camera on the block's -x face
         if (mode == M_ADJ && nee_camera_on_face(d) && sc->spline_min >= 1.f) {
-            if (!ctx->d_cut) HIP_TRY(ctx, hipMalloc(&ctx->d_cut, 8 * sizeof(int)));
-            hipLaunchKernelGGL(k_cut_init, dim3(1), dim3(64), 0, ctx->cur, ctx->d_cut);
-            const size_t ng = (size_t)X * Y * Z, nc = (size_t)sc->sN[0] * sc->sN[1] * sc->sN[2];
-            hipLaunchKernelGGL(k_gbox, dim3((unsigned)std::min<size_t>((ng + 255) / 256, 4096)), dim3(256), 0,
-                               ctx->cur, G, X, Y, Z, ctx->d_cut);
+            rc = nee_cut_begin(ctx, G, X, Y, Z, 0.f);
+            if (rc) return rc;
+            const size_t nc = (size_t)sc->sN[0] * sc->sN[1] * sc->sN[2];
             hipLaunchKernelGGL(k_spline_dmax, dim3((unsigned)std::min<size_t>((nc + 255) / 256, 4096)), dim3(256), 0,
                                ctx->cur, sc->d_spline, sc->sN[0], sc->sN[1], sc->sN[2],
                                reinterpret_cast<unsigned *>(ctx->d_cut + 4));
             HIP_TRY(ctx, hipGetLastError());
+            A.nee_cut = ctx->d_cut;
+        }
+    }
+    if (d.field_kind == SIMPLER_FIELD_SOURCES && mode == M_ADJ && n > 0 && nee_camera_on_face(d)) {
+        // point-source fields: the host's bound on |grad n|
+        const double gb = sources_grad_bound(sc, nullptr);
+        if (gb >= 0.0 && gb < 1e30) {
+            rc = nee_cut_begin(ctx, G, X, Y, Z, (float)gb);
+            if (rc) return rc;
             A.nee_cut = ctx->d_cut;
         }
     }
@@ -3131,6 +3185,7 @@ struct simpler_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool counted = false;  // registered with ctx->live / base->batches
+    double nee_g = -1.0;   // point-source patterns: bound on |grad n| over all patterns (< 0: none)
 };
 
 // Each configuration's image pointer in the batch's device parameters
@@ -3389,6 +3444,14 @@ int simpler_batch_create_sources(simpler_scene *base, int n_cfg, const float *am
     b->Z = Z;
     b->total = prefix[n_cfg];
     b->field = SIMPLER_FIELD_SOURCES;
+    for (int i = 0; i < n_cfg; ++i) {  // the adjoint's nee_skip takes the largest
+        const double g = sources_grad_bound(base, amp + (size_t)i * N);
+        if (g < 0.0) {
+            b->nee_g = -1.0;
+            break;
+        }
+        b->nee_g = std::max(b->nee_g, g);
+    }
     if (hipMalloc(&b->d_src, src.size() * sizeof(float4)) != hipSuccess ||
         hipMalloc(&b->d_phase, src.size() * sizeof(float)) != hipSuccess ||
         hipMalloc(&b->d_cfgs, vars.size() * sizeof(Var)) != hipSuccess ||
@@ -3473,6 +3536,14 @@ int simpler_batch_adjoint(simpler_batch *b, const float *dF_dimages, float *grad
                            grads, n, 0.0f);
     }
     rc = SIMPLER_OK;
+    if (b->total > 0 && b->nee_g >= 0.0 && b->nee_g < 1e30 && nee_camera_on_face(sc->desc)) {
+        // next-event sub-paths that cannot reach any pattern's objective
+        // are not traced (nee_skip: the union of the supports, the largest
+        // bound)
+        rc = nee_cut_begin(ctx, dF_dimages, b->X, b->Y, b->n_cfg * b->Z, (float)b->nee_g);
+        if (rc) return rc;
+        A.nee_cut = ctx->d_cut;
+    }
     if (b->total > 0) {
         const dim3 grid(std::max(1, ctx->num_cu * kAdjBlocksPerCU));
         if (ctx->adj_sched == 1) {
