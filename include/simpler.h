@@ -497,8 +497,8 @@ int simpler_ctx_get_adjoint_schedule(simpler_ctx *ctx, int *schedule, int64_t *q
  * the bound allows) or meets the sensor face outside the image.  The image,
  * the splat counts and every draw of the photon are those of the full trace;
  * er_steps and face_events count the work done and drop.  Record renders and
- * adjoints are not affected (image adjoints of spline and point-source
- * fields skip by their objective's support in any case). */
+ * adjoints are not affected (image adjoints of every field skip by their
+ * objective's support in any case). */
 int simpler_ctx_set_next_event_skip(simpler_ctx *ctx, int enable);
 int simpler_ctx_get_next_event_skip(simpler_ctx *ctx, int *enable);
 

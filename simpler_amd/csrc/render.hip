@@ -103,7 +103,8 @@ struct RenderArgs {
     // image adjoint: the objective's support (pixel box over all planes:
     // min x, max x, min y, max y; max x < 0 when it is empty), the largest
     // spline coefficient difference along each axis (float bits) and a
-    // bound on |grad n| from the host (float bits; point-source fields), for
+    // bound on |grad n| from the host (float bits; Bessel and point-source
+    // fields), for
     // nee_skip; null: every next-event sub-path is traced
     const int *nee_cut;
     // forward renders of spline fields (nee_cut null): nee_fwd != 0 skips
@@ -733,14 +734,13 @@ __device__ __forceinline__ float fresnel_upper(float i0, float i1, float t0, flo
 }
 
 // Can the next-event sub-path that starts at the lane's vertex in direction
-// d (unit) reach a pixel with a nonzero objective weight (image adjoint of a
-// spline or point-source field), or the image at all (forward render of a
-// spline field)?  The field bends the path by at most G s^2 / 2 and turns it
+// d (unit) reach a pixel with a nonzero objective weight (image adjoint), or
+// the image at all (forward render of a spline field)?  The field bends the path by at most G s^2 / 2 and turns it
 // by at most G s over a length s, with G >= |grad n| >= |grad n| / n: for
 // splines (n >= 1) |dn/dx_a| <= res_a max|c_{i+1} - c_i| along a, the cubic
 // B-spline's derivative being a convex combination of coefficient
-// differences; point-source fields and forward renders take the host's
-// bound (cut[7], nee_g).  The path is replayed as a polyline from face to face (the
+// differences; Bessel and point-source fields and forward renders take the
+// host's bound (cut[7], nee_g).  The path is replayed as a polyline from face to face (the
 // camera sits on the block's -x face, where next-event paths leave, looking
 // along -x): every face it meets must lie more than the bound inside the
 // other faces' planes (a path at distance up to the bound from the line then
@@ -1343,7 +1343,8 @@ __global__ void k_spline_dmax(const float *c, int N0, int N1, int N2, unsigned *
 }
 
 // Initial values of the k_gbox / k_spline_dmax accumulators, and a bound g
-// on |grad n| that the host knows (point-source fields; 0 otherwise).
+// on |grad n| that the host knows (Bessel and point-source fields; 0
+// otherwise).
 __global__ void k_cut_init(int *cut, float g) {
     const int init[8] = {0x7fffffff, -1, 0x7fffffff, -1, 0, 0, 0, __float_as_int(g)};
     if (threadIdx.x < 8) cut[threadIdx.x] = init[threadIdx.x];
@@ -2534,6 +2535,19 @@ static double sources_grad_bound(const simpler_scene *sc, const float *amp) {
     return g / std::min(1.0, nmin);
 }
 
+// The same for a Bessel field: |grad n| <= 2 a k, a = |n_max| min(1,
+// |n_clip / n_max|) the amplitude times its largest duty scaling and k the
+// wavenumber (|J_m'| <= 1 and m |J_m(x)| / x <= 1, halves of
+// J_{m-1} -+ J_{m+1}; the gradient clamped near the axis is at most sqrt 2
+// times more), divided by the least index n_o - a when that is below 1.
+static double bessel_grad_bound(const DevScene &S) {
+    const double a = std::fabs((double)S.amp) * std::min(1.0, std::fabs((double)S.n_max_scaling));
+    const double nmin = (double)S.n_o - a;
+    const double g = 2.0 * a * std::fabs((double)S.k_r);
+    if (!(nmin > 0.0) || !std::isfinite(g)) return -1.0;
+    return g / std::min(1.0, nmin);
+}
+
 // nee_skip's geometry: the camera on the block's -x face looking along -x
 // (where the next-event hemisphere points) and no sensor lens (the splat is
 // then where the path meets the face).
@@ -3082,9 +3096,10 @@ static int adjoint_impl(simpler_scene *sc, int mode, int64_t n, int64_t n_total,
             A.nee_cut = ctx->d_cut;
         }
     }
-    if (d.field_kind == SIMPLER_FIELD_SOURCES && mode == M_ADJ && n > 0 && nee_camera_on_face(d)) {
-        // point-source fields: the host's bound on |grad n|
-        const double gb = sources_grad_bound(sc, nullptr);
+    if (d.field_kind != SIMPLER_FIELD_SPLINE && mode == M_ADJ && n > 0 && nee_camera_on_face(d)) {
+        // point-source and Bessel fields: the host's bound on |grad n|
+        const double gb =
+            d.field_kind == SIMPLER_FIELD_SOURCES ? sources_grad_bound(sc, nullptr) : bessel_grad_bound(S);
         if (gb >= 0.0 && gb < 1e30) {
             rc = nee_cut_begin(ctx, G, X, Y, Z, (float)gb);
             if (rc) return rc;

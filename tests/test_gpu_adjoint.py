@@ -416,14 +416,20 @@ def test_untraced_next_event_paths_leave_the_objective_unchanged(gpu_ctx):
     assert F0 == 0.0 and float(g0.abs().max()) == 0.0
 
 
-def test_untraced_next_event_paths_leave_the_point_source_objective_unchanged(gpu_ctx):
+@pytest.mark.parametrize("kind", ["sources", "bessel"])
+def test_untraced_next_event_paths_leave_the_point_source_objective_unchanged(gpu_ctx, kind):
     """The same skip for the 16-element travelling-wave ring (1 MHz, sources
-    on a 10 mm ring around the 10 mm block) with a 1 mm aperture objective:
-    the bound on |grad n| comes from the amplitudes and the sources'
-    distances to the block.  F equals sum G * render over the same photons
-    and the re-trace takes fewer forward steps than the full trace."""
-    d, tex, res, src = scenes.travelling_wave_cube()
-    sc = engine.Scene(gpu_ctx, d, tex, sources=src)
+    on a 10 mm ring around the 10 mm block; the bound on |grad n| from the
+    amplitudes and the sources' distances to the block) and for the Bessel
+    cube (m = 2, dn 2e-5, 848 kHz; the bound 2 n_max k), each with a 1 mm
+    aperture objective.  F equals sum G * render over the same photons and
+    the re-trace takes fewer forward steps than the full trace."""
+    if kind == "sources":
+        d, tex, res, src = scenes.travelling_wave_cube()
+        sc = engine.Scene(gpu_ctx, d, tex, sources=src)
+    else:
+        d, tex, res = scenes.bessel_cube(step=1e-5, n_max=2e-5, mode=2, res=(64, 64, 32))
+        sc = engine.Scene(gpu_ctx, d, tex)
     G = torch.tensor(scenes.aperture_weights(res), device="cuda")
     n = 1 << 16
     with full_trace(gpu_ctx):
@@ -431,7 +437,7 @@ def test_untraced_next_event_paths_leave_the_point_source_objective_unchanged(gp
     _, F, ast = sc.adjoint_render(G, n, seed=23)
     sc.close()
     Fr = float((img.double() * G.double()).sum())
-    print(f"sources: F adjoint {F:.9g} render {Fr:.9g}; forward steps adjoint {ast['er_steps']} "
+    print(f"{kind}: F adjoint {F:.9g} render {Fr:.9g}; forward steps adjoint {ast['er_steps']} "
           f"render {rst['er_steps']}", flush=True)
     assert Fr > 0
     assert abs(F - Fr) <= 5e-6 * abs(Fr)

@@ -11,7 +11,9 @@ everywhere in the block.  The bounds, as the engine's host code forms them:
   differences), G = |(res_a dmax_a)_a|;
 * point sources, n = n_o + sum_i a_i sum_c w_c cos(c k r_i + phi_i) / r_i:
   |grad n| <= sum_i |a_i| sum_c |w_c| (1 / d_i^2 + c k / d_i), d_i the
-  source's distance from the block, and n >= n_o - sum_i |a_i| sum_c |w_c| / d_i.
+  source's distance from the block, and n >= n_o - sum_i |a_i| sum_c |w_c| / d_i;
+* Bessel fields: |grad n| <= 2 a k, a the amplitude times its largest duty
+  scaling, k the wavenumber.
 
 Checked at random points of the block and at the block's points nearest to
 each source (where a source's term peaks), for the bench's ring and for
@@ -86,6 +88,32 @@ def test_point_source_bound_is_tight_for_one_source():
         gmax = max(gmax, float(np.linalg.norm(g, axis=1).max()))
     print(f"one source: max |grad n| {gmax:.5g} of the bound {G:.5g}", flush=True)
     assert 0.9 * G <= gmax <= G
+
+
+def test_bessel_gradient_bound_holds_over_the_block():
+    """|grad n| <= 2 a k for the Bessel field, a = n_max min(1, n_clip /
+    n_max), at random points, near the axis (where the gradient is clamped)
+    and for every duty scaling, orders 0-3 and both axis forms."""
+    for mode in (0, 1, 2, 3):
+        for axis in ("z", "x"):
+            d, tex, _ = scenes.bessel_cube(step=5e-6, n_max=2e-5, mode=mode, axis=axis)
+            osc = O.OracleScene(d, tex)
+            a = abs(d.n_max) * min(1.0, abs(d.n_clip / d.n_max))
+            k = 2.0 * math.pi * d.f_u / d.speed_u
+            G = 2.0 * a * k
+            pts, _, _ = _block_points(d, 20_000, seed=mode)
+            ax = np.array(d.axis_uz, np.float64)
+            t = np.random.default_rng(7).uniform(-0.005, 0.005, (4000, 1))
+            off = np.random.default_rng(8).normal(size=(4000, 3)) * 1e-7
+            near_axis = t * ax + off - (off @ ax)[:, None] * ax  # within a few 1e-7 m of the axis
+            gmax = 0.0
+            for scaling in (1.0, -1.0, 0.3):
+                n, g = osc.field(np.concatenate([pts, near_axis]), scaling=scaling)
+                gmax = max(gmax, float(np.linalg.norm(g, axis=1).max()))
+                assert n.min() >= d.n_o - a - 1e-12
+            print(f"bessel m={mode} axis {axis}: max |grad n| {gmax:.4g} of the bound {G:.4g}", flush=True)
+            assert gmax <= G, (mode, axis)
+            assert gmax >= 0.1 * G  # not vacuous (m = 0: max |J_1| / 2 = 0.29)
 
 
 def test_spline_gradient_bound_holds_over_the_block():
